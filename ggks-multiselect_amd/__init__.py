@@ -1,0 +1,14 @@
+"""ggks-multiselect_amd: MI355X-native selection (bucket multiselect, MSD radix
+multiselect, top-K) behind the C ABI of libggks.so.
+
+The directory name holds a dash, so it is imported by path:
+``load_package()`` in __graft_entry__.py registers it as
+``ggks_multiselect_amd``.
+"""
+from .api import (Context, DeviceArray, DeviceView, GGKSError,  # noqa: F401
+                  RadixSelectWrapper, bucketMultiselectWrapper, multiselect,
+                  multiselect_async, randomizedTopkSelectWrapper, scratch_bytes,
+                  sync, topk)
+from . import workloads  # noqa: F401
+
+__version__ = "0.1"

@@ -1,0 +1,98 @@
+"""Loader for libggks.so (the HIP library) and its C ABI signatures.
+
+The library is built in-tree by ``build.py`` (hipcc, gfx950).  There is no
+fallback: if the shared object is missing, ``lib()`` raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libggks.so")
+
+# ggks.h status codes
+OK, EINVAL, ENOMEM, EDEVICE, EUNSUPPORTED = 0, -1, -2, -3, -4
+F32, F64, U32, I32 = 0, 1, 2, 3
+KIND_BUCKET, KIND_RADIX, KIND_TOPK = 0, 1, 2
+
+# every symbol include/ggks.h declares (checked by the CPU test suite)
+EXPORTS = [
+    "ggks_ctx_create", "ggks_ctx_destroy", "ggks_scratch_bytes", "ggks_last_error",
+    "ggks_version", "ggks_bucket_multiselect", "ggks_bucket_multiselect_f32",
+    "ggks_bucket_multiselect_f64", "ggks_bucket_multiselect_u32",
+    "ggks_bucket_multiselect_i32", "ggks_radix_multiselect", "ggks_topk",
+    "ggks_bucket_multiselect_async", "ggks_radix_multiselect_async", "ggks_topk_async",
+    "ggks_ctx_reserve", "ggks_ctx_check", "ggks_ctx_stream", "ggks_dev_alloc",
+    "ggks_dev_free", "ggks_h2d", "ggks_d2h", "ggks_d2d", "ggks_sync",
+    "ggks_device_count", "ggks_set_device", "ggks_dev_fill",
+]
+
+
+class GGKSError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"ggks error {code}: {msg}")
+        self.code = code
+
+
+class Opts(ctypes.Structure):
+    """ggks_opts (include/ggks.h)."""
+    _fields_ = [("blocks", ctypes.c_int32), ("threads", ctypes.c_int32),
+                ("buckets_log2", ctypes.c_int32), ("destructive", ctypes.c_int32),
+                ("seed", ctypes.c_uint64), ("radix_bits", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 5)]
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FileNotFoundError(
+            f"{LIB_PATH} is not built; run `python __graft_entry__.py` or "
+            "`python ggks-multiselect_amd/build.py` first")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i64, i32, sz = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_size_t
+    optp = ctypes.POINTER(Opts)
+    sig = {
+        "ggks_ctx_create": ([ctypes.POINTER(vp), i32, vp], i32),
+        "ggks_ctx_destroy": ([vp], i32),
+        "ggks_scratch_bytes": ([i32, i32, i64, i64], sz),
+        "ggks_last_error": ([], ctypes.c_char_p),
+        "ggks_version": ([], ctypes.c_char_p),
+        "ggks_bucket_multiselect": ([vp, i32, vp, i64, vp, i64, vp, optp], i32),
+        "ggks_radix_multiselect": ([vp, i32, vp, i64, vp, i64, vp, optp], i32),
+        "ggks_topk": ([vp, i32, vp, i64, i64, vp, optp], i32),
+        "ggks_bucket_multiselect_async": ([vp, i32, vp, i64, vp, i64, vp, optp], i32),
+        "ggks_radix_multiselect_async": ([vp, i32, vp, i64, vp, i64, vp, optp], i32),
+        "ggks_topk_async": ([vp, i32, vp, i64, i64, vp, optp], i32),
+        "ggks_ctx_reserve": ([vp, i32, i32, i64, i64], i32),
+        "ggks_ctx_check": ([vp], i32),
+        "ggks_ctx_stream": ([vp], vp),
+        "ggks_dev_alloc": ([ctypes.POINTER(vp), sz], i32),
+        "ggks_dev_free": ([vp], i32),
+        "ggks_h2d": ([vp, vp, sz], i32),
+        "ggks_d2h": ([vp, vp, sz], i32),
+        "ggks_d2d": ([vp, vp, sz], i32),
+        "ggks_sync": ([], i32),
+        "ggks_device_count": ([ctypes.POINTER(i32)], i32),
+        "ggks_set_device": ([i32], i32),
+        "ggks_dev_fill": ([vp, i32, sz], i32),
+    }
+    for t in ("f32", "f64", "u32", "i32"):
+        sig[f"ggks_bucket_multiselect_{t}"] = ([vp, i64, vp, ctypes.c_int32, vp, optp], i32)
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def check(rc: int) -> None:
+    if rc != OK:
+        msg = lib().ggks_last_error()
+        raise GGKSError(rc, msg.decode() if msg else "")

@@ -1,0 +1,207 @@
+"""Python face of libggks.so, mirroring the reference's wrapper interface.
+
+Reference entry points and their counterparts here:
+
+=============================================================  ==================================
+reference                                                      here
+=============================================================  ==================================
+BucketMultiselect::bucketMultiselectWrapper                    :func:`bucketMultiselectWrapper`
+  (bucketMultiselect.cu:837-871)
+RadixSelect::RadixSelectWrapper (merrillSelect.cu:210-259)     :func:`RadixSelectWrapper`
+randomizedTopkSelectWrapper (randomizedTopkSelect.cu:210-213)  :func:`randomizedTopkSelectWrapper`
+timeBucketTopkselect's k = 1..K call                           :func:`topk`
+  (multiselectTimingFunctions.cu:204-231)
+=============================================================  ==================================
+
+Semantics follow the reference: k is the 1-based rank of the k-th largest
+element, outputs follow the caller's k order, top-K is descending.  Unlike the
+reference, the input vector is never modified and errors raise
+:class:`GGKSError` instead of printing and returning the success value
+(bucketMultiselect.cu:30-32).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import GGKSError, Opts, check, lib
+
+_DTYPE_CODE = {np.dtype(np.float32): _lib.F32, np.dtype(np.float64): _lib.F64,
+               np.dtype(np.uint32): _lib.U32, np.dtype(np.int32): _lib.I32}
+
+
+def dtype_code(dtype) -> int:
+    d = np.dtype(dtype)
+    if d not in _DTYPE_CODE:
+        raise TypeError(f"unsupported dtype {d}; use float32, float64, uint32 or int32")
+    return _DTYPE_CODE[d]
+
+
+class DeviceArray:
+    """A device buffer owned by this process (allocated through libggks)."""
+
+    def __init__(self, n: int, dtype):
+        self.dtype = np.dtype(dtype)
+        self.n = int(n)
+        self.nbytes = self.n * self.dtype.itemsize
+        p = ctypes.c_void_p()
+        check(lib().ggks_dev_alloc(ctypes.byref(p), max(self.nbytes, 1)))
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = cls(a.size, a.dtype)
+        if a.size:
+            check(lib().ggks_h2d(d.ptr, a.ctypes.data, d.nbytes))
+        return d
+
+    def to_host(self) -> np.ndarray:
+        out = np.empty(self.n, self.dtype)
+        if self.n:
+            check(lib().ggks_d2h(out.ctypes.data, self.ptr, self.nbytes))
+        return out
+
+    def view(self, offset: int, n: int) -> "DeviceView":
+        return DeviceView(self, offset, n)
+
+    def free(self) -> None:
+        if self.ptr:
+            lib().ggks_dev_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def __len__(self):
+        return self.n
+
+
+class DeviceView:
+    """A window [offset, offset + n) of a DeviceArray (no ownership)."""
+
+    def __init__(self, base: DeviceArray, offset: int, n: int):
+        if offset < 0 or n < 0 or offset + n > base.n:
+            raise ValueError("view out of range")
+        self.base, self.dtype, self.n = base, base.dtype, int(n)
+        self.ptr = base.ptr + offset * base.dtype.itemsize
+        self.nbytes = self.n * self.dtype.itemsize
+
+    def __len__(self):
+        return self.n
+
+
+class Context:
+    """A ggks_ctx: its own stream (None = null stream) and scratch arena."""
+
+    def __init__(self, device: int = 0, stream: int | None = None):
+        p = ctypes.c_void_p()
+        check(lib().ggks_ctx_create(ctypes.byref(p), int(device), stream))
+        self.handle = p.value
+
+    def close(self):
+        if self.handle:
+            lib().ggks_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _opts(blocks=0, threads=0, buckets_log2=0, seed=0, radix_bits=0):
+    o = Opts()
+    o.blocks, o.threads, o.buckets_log2 = int(blocks), int(threads), int(buckets_log2)
+    o.seed, o.radix_bits = int(seed), int(radix_bits)
+    return o
+
+
+def _k_array(kvals) -> np.ndarray:
+    k = np.asarray(kvals)
+    if k.size and (k.min() < 1 or k.max() > 0xFFFFFFFF):
+        raise GGKSError(_lib.EINVAL, "k outside [1, 2^32)")
+    return np.ascontiguousarray(k, dtype=np.uint32)
+
+
+def multiselect(dvec, kvals, ctx: Context | None = None, kind: str = "bucket",
+                **opt) -> np.ndarray:
+    """Values of the k-th largest elements of device vector `dvec`, in k order."""
+    k = _k_array(kvals)
+    out = np.empty(k.size, dvec.dtype)
+    fn = lib().ggks_bucket_multiselect if kind == "bucket" else lib().ggks_radix_multiselect
+    o = _opts(**opt)
+    check(fn(ctx.handle if ctx else None, dtype_code(dvec.dtype), dvec.ptr, len(dvec),
+             k.ctypes.data, k.size, out.ctypes.data, ctypes.byref(o)))
+    return out
+
+
+def topk(dvec, K: int, ctx: Context | None = None, **opt) -> np.ndarray:
+    """The K largest values of `dvec`, descending."""
+    out = np.empty(int(K), dvec.dtype)
+    o = _opts(**opt)
+    check(lib().ggks_topk(ctx.handle if ctx else None, dtype_code(dvec.dtype), dvec.ptr,
+                          len(dvec), int(K), out.ctypes.data, ctypes.byref(o)))
+    return out
+
+
+def multiselect_async(dvec, d_k: DeviceArray, d_out: DeviceArray, ctx: Context,
+                      kind: str = "bucket", **opt) -> None:
+    """Stream-ordered select: device k list in, device outputs out."""
+    fn = (lib().ggks_bucket_multiselect_async if kind == "bucket"
+          else lib().ggks_radix_multiselect_async)
+    o = _opts(**opt)
+    check(fn(ctx.handle, dtype_code(dvec.dtype), dvec.ptr, len(dvec), d_k.ptr, len(d_k),
+             d_out.ptr, ctypes.byref(o)))
+
+
+def scratch_bytes(kind: str, dtype, n: int, num_k: int) -> int:
+    code = {"bucket": _lib.KIND_BUCKET, "radix": _lib.KIND_RADIX, "topk": _lib.KIND_TOPK}[kind]
+    return int(lib().ggks_scratch_bytes(code, dtype_code(dtype), int(n), int(num_k)))
+
+
+def sync() -> None:
+    check(lib().ggks_sync())
+
+
+# ----------------------------------------------- reference-named wrappers
+def bucketMultiselectWrapper(d_vector, length: int, kVals_ori, numKs: int, outputs,
+                             blocks: int = 0, threads: int = 0):
+    """bucketMultiselect.cu:837-871: fills host array `outputs[numKs]` with the
+    kVals_ori[i]-th largest values of the first `length` elements of
+    `d_vector`.  Returns 1 as the reference does."""
+    if length < 1 or length > len(d_vector):
+        raise GGKSError(_lib.EINVAL, "length outside [1, len(d_vector)]")
+    vec = d_vector if length == len(d_vector) else DeviceView(
+        d_vector if isinstance(d_vector, DeviceArray) else d_vector.base, 0, length)
+    k = _k_array(np.asarray(kVals_ori)[:numKs])
+    res = multiselect(vec, k, blocks=blocks, threads=threads)
+    outputs[:numKs] = res
+    return 1
+
+
+def RadixSelectWrapper(d_vec, size: int, k: int):
+    """merrillSelect.cu:210-259: the k-th largest of the first `size`
+    elements (input left intact)."""
+    vec = d_vec if size == len(d_vec) else DeviceView(
+        d_vec if isinstance(d_vec, DeviceArray) else d_vec.base, 0, size)
+    return multiselect(vec, [k], kind="radix")[0]
+
+
+def randomizedTopkSelectWrapper(lst, size: int, k: int) -> np.ndarray:
+    """randomizedTopkSelect.cu:210-213: the k largest values, descending, as a
+    new host array (the reference returns a malloc'd array)."""
+    vec = lst if size == len(lst) else DeviceView(
+        lst if isinstance(lst, DeviceArray) else lst.base, 0, size)
+    return topk(vec, k)
+
+
+__all__ = ["DeviceArray", "DeviceView", "Context", "GGKSError", "multiselect", "topk",
+           "multiselect_async", "scratch_bytes", "sync", "bucketMultiselectWrapper",
+           "RadixSelectWrapper", "randomizedTopkSelectWrapper", "dtype_code"]

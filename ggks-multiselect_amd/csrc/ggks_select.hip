@@ -1,0 +1,456 @@
+// ggks_select.hip -- host orchestration and the C ABI of libggks.so.
+//
+// One call = one plan: a table of top-level segments (a single vector is one
+// segment), their chunks and reduce items, laid out in the context's arena.
+// The plan tables go to the device with one async copy from pinned memory,
+// then the level kernels run back to back on the context's stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "ggks.h"
+#include "ggks_kernels.hpp"
+#include "ggks_plan.hpp"
+
+namespace ggks {
+
+// ------------------------------------------------------------------ errors
+static thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                          \
+  do {                                                                         \
+    hipError_t e_ = (expr);                                                    \
+    if (e_ != hipSuccess)                                                      \
+      return fail(GGKS_EDEVICE, "%s failed: %s (%s:%d)", #expr,                \
+                  hipGetErrorString(e_), __FILE__, __LINE__);                  \
+  } while (0)
+
+// ------------------------------------------------------------------ context
+}  // namespace ggks
+
+struct ggks_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  char* arena = nullptr;
+  size_t arena_cap = 0;
+  char* pinned = nullptr;
+  size_t pinned_cap = 0;
+  uint32_t* status = nullptr;  // device word
+  std::mutex mu;
+};
+
+namespace ggks {
+
+static int ensure_arena(ggks_ctx* c, size_t bytes) {
+  if (bytes <= c->arena_cap) return GGKS_OK;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (c->arena) HIP_TRY(hipFree(c->arena));
+  c->arena = nullptr;
+  c->arena_cap = 0;
+  size_t cap = bytes + bytes / 8;
+  if (hipMalloc(&c->arena, cap) != hipSuccess) {
+    c->arena = nullptr;
+    (void)hipGetLastError();
+    return fail(GGKS_ENOMEM, "cannot allocate %zu bytes of device scratch", cap);
+  }
+  c->arena_cap = cap;
+  return GGKS_OK;
+}
+
+static int ensure_pinned(ggks_ctx* c, size_t bytes) {
+  if (bytes <= c->pinned_cap) return GGKS_OK;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (c->pinned) HIP_TRY(hipHostFree(c->pinned));
+  c->pinned = nullptr;
+  c->pinned_cap = 0;
+  size_t cap = std::max<size_t>(bytes + bytes / 8, 1 << 16);
+  if (hipHostMalloc(&c->pinned, cap, hipHostMallocDefault) != hipSuccess) {
+    c->pinned = nullptr;
+    (void)hipGetLastError();
+    return fail(GGKS_ENOMEM, "cannot allocate %zu bytes of pinned memory", cap);
+  }
+  c->pinned_cap = cap;
+  return GGKS_OK;
+}
+
+static int ctx_init(ggks_ctx* c, int device, hipStream_t stream) {
+  c->device = device;
+  c->stream = stream;
+  HIP_TRY(hipSetDevice(device));
+  HIP_TRY(hipMalloc(&c->status, 256));
+  HIP_TRY(hipMemset(c->status, 0, 256));
+  return GGKS_OK;
+}
+
+static std::mutex g_default_mu;
+static ggks_ctx* g_default[64];
+
+static int default_ctx(ggks_ctx** out) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return fail(GGKS_EINVAL, "device %d out of range", dev);
+  std::lock_guard<std::mutex> lk(g_default_mu);
+  if (!g_default[dev]) {
+    ggks_ctx* c = new ggks_ctx();
+    int rc = ctx_init(c, dev, nullptr);
+    if (rc) { delete c; return rc; }
+    g_default[dev] = c;
+  }
+  *out = g_default[dev];
+  return GGKS_OK;
+}
+
+// ----------------------------------------------------------- kernel launch
+template <typename K>
+static void set_lds_limit(K kernel, size_t bytes) {
+  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+template <typename KT, int C>
+static int launch_level(ggks_ctx* c, const Plan& p, const KT* d_src, KT* d_out) {
+  char* A = c->arena;
+  const SegDesc* segs = reinterpret_cast<const SegDesc*>(A + p.off_segs);
+  SegState* states = reinterpret_cast<SegState*>(A + p.off_states);
+  const Chunk* chunks = reinterpret_cast<const Chunk*>(A + p.off_chunks);
+  const ReduceItem* items = reinterpret_cast<const ReduceItem*>(A + p.off_items);
+  KT* samples = reinterpret_cast<KT*>(A + p.off_samples);
+  uint32_t* lut = reinterpret_cast<uint32_t*>(A + p.off_lut);
+  uint32_t* rows = reinterpret_cast<uint32_t*>(A + p.off_rows);
+  uint32_t* tot = reinterpret_cast<uint32_t*>(A + p.off_tot);
+  uint32_t* bits = reinterpret_cast<uint32_t*>(A + p.off_bits);
+  uint32_t* wbase = reinterpret_cast<uint32_t*>(A + p.off_wbase);
+  const Query* q = reinterpret_cast<const Query*>(A + p.off_q);
+  uint32_t* qb = reinterpret_cast<uint32_t*>(A + p.off_qb);
+  uint32_t* qr = reinterpret_cast<uint32_t*>(A + p.off_qr);
+  Child* kids = reinterpret_cast<Child*>(A + p.off_children);
+  Query* cq = reinterpret_cast<Query*>(A + p.off_cq);
+  KT* red = reinterpret_cast<KT*>(A + p.off_red);
+  hipStream_t s = c->stream;
+  const uint32_t nseg = (uint32_t)p.segs.size();
+
+  if (p.any_sample) {
+    dim3 g(nseg, (p.max_nsample + kThreads - 1) / kThreads);
+    hipLaunchKernelGGL((k_sample<KT, C>), g, dim3(kThreads), 0, s, d_src, segs, samples, p.seed);
+  }
+  hipLaunchKernelGGL((k_lut<KT>), dim3(nseg), dim3(kThreads), 0, s, segs, samples, states, lut);
+
+  const size_t lds_count = ((size_t)p.bmax_max + (1u << p.pbits_max)) * 4;
+  set_lds_limit(k_count<KT, C>, lds_count);
+  hipLaunchKernelGGL((k_count<KT, C>), dim3((uint32_t)p.chunks.size()), dim3(kThreads),
+                     lds_count, s, d_src, segs, states, chunks, lut, rows);
+  hipLaunchKernelGGL(k_reduce, dim3((uint32_t)p.items.size()), dim3(256), 0, s, segs, states,
+                     items, rows, tot);
+  const size_t lds_findk =
+      ((size_t)p.bmax_max + (1u << p.pbits_max) + 2 * ((p.bmax_max + 31) / 32) + 32) * 4;
+  set_lds_limit(k_findk<KT, C>, lds_findk);
+  hipLaunchKernelGGL((k_findk<KT, C>), dim3(nseg), dim3(kThreads), lds_findk, s, segs, states,
+                     lut, tot, q, qb, qr, kids, cq, bits, wbase, d_out);
+  const size_t lds_compact =
+      ((size_t)(1u << p.pbits_max) + 2 * ((p.bmax_max + 31) / 32) + kCursorCap) * 4;
+  set_lds_limit(k_compact<KT, C>, lds_compact);
+  hipLaunchKernelGGL((k_compact<KT, C>), dim3((uint32_t)p.chunks.size()), dim3(kThreads),
+                     lds_compact, s, d_src, segs, states, chunks, lut, rows, bits, wbase, kids,
+                     red);
+  set_lds_limit(k_finish<KT, C>, kFinishLdsTotal);
+  const uint32_t fgrid = std::min<uint32_t>(p.nq, 2048);
+  hipLaunchKernelGGL((k_finish<KT, C>), dim3(fgrid), dim3(kThreads), kFinishLdsTotal, s, d_src,
+                     red, kids, p.nq, cq, d_out);
+  HIP_TRY(hipGetLastError());
+  return GGKS_OK;
+}
+
+// Small vector: one child on the input, finished in LDS.
+template <typename KT, int C>
+static int launch_direct(ggks_ctx* c, const Plan& p, const KT* d_src, KT* d_out) {
+  char* A = c->arena;
+  const Child* kids = reinterpret_cast<const Child*>(A + p.off_children);
+  const Query* q = reinterpret_cast<const Query*>(A + p.off_q);
+  set_lds_limit(k_finish<KT, C>, kFinishLdsTotal);
+  hipLaunchKernelGGL((k_finish<KT, C>), dim3(1), dim3(kThreads), kFinishLdsTotal, c->stream,
+                     d_src, (const KT*)nullptr, kids, 1u, q, d_out);
+  HIP_TRY(hipGetLastError());
+  return GGKS_OK;
+}
+
+template <typename KT, int C>
+static int launch_typed(ggks_ctx* c, const Plan& p, const void* d_src, void* d_out) {
+  if (p.direct)
+    return launch_direct<KT, C>(c, p, (const KT*)d_src, (KT*)d_out);
+  return launch_level<KT, C>(c, p, (const KT*)d_src, (KT*)d_out);
+}
+
+static int launch(ggks_ctx* c, int dtype, const Plan& p, const void* d_src, void* d_out) {
+  switch (dtype) {
+    case GGKS_F32: return launch_typed<uint32_t, CONV_F>(c, p, d_src, d_out);
+    case GGKS_F64: return launch_typed<uint64_t, CONV_F>(c, p, d_src, d_out);
+    case GGKS_U32: return launch_typed<uint32_t, CONV_U>(c, p, d_src, d_out);
+    case GGKS_I32: return launch_typed<uint32_t, CONV_I>(c, p, d_src, d_out);
+  }
+  return fail(GGKS_EINVAL, "bad dtype %d", dtype);
+}
+
+// -------------------------------------------------------------- one select
+enum class KSource { HOST, DEVICE, TOPK };
+
+static int check_common(int dtype, const void* d_vec, int64_t n, int64_t num_k) {
+  if (dtype < 0 || dtype > 3) return fail(GGKS_EINVAL, "bad dtype %d", dtype);
+  if (!d_vec) return fail(GGKS_EINVAL, "null vector");
+  if (n < 1 || n > (int64_t)0xFFFFFFFFll) return fail(GGKS_EINVAL, "n=%lld outside [1, 2^32)", (long long)n);
+  if (num_k < 0 || num_k > (int64_t)0x7FFFFFFFll) return fail(GGKS_EINVAL, "num_k=%lld", (long long)num_k);
+  return GGKS_OK;
+}
+
+static int select_one(ggks_ctx* c, int kind, int dtype, const void* d_vec, int64_t n,
+                      const uint32_t* k, int64_t num_k, KSource ksrc, void* out,
+                      bool host_out, const ggks_opts* opt) {
+  int rc = check_common(dtype, d_vec, n, num_k);
+  if (rc) return rc;
+  if (ksrc == KSource::HOST) {
+    if (num_k > 0 && (!k || !out)) return fail(GGKS_EINVAL, "null k or output array");
+    for (int64_t i = 0; i < num_k; ++i)
+      if (k[i] < 1 || (int64_t)k[i] > n)
+        return fail(GGKS_EINVAL, "k[%lld]=%u outside [1, %lld]", (long long)i, k[i], (long long)n);
+  } else if (ksrc == KSource::DEVICE) {
+    if (num_k > 0 && (!k || !out)) return fail(GGKS_EINVAL, "null k or output array");
+  } else {
+    if (num_k < 1 || num_k > n) return fail(GGKS_EINVAL, "K=%lld outside [1, n]", (long long)num_k);
+    if (!out) return fail(GGKS_EINVAL, "null output array");
+  }
+  if (num_k == 0) return GGKS_OK;
+  ggks_opts o = {};
+  if (opt) o = *opt;
+  std::lock_guard<std::mutex> lk(c->mu);
+  HIP_TRY(hipSetDevice(c->device));
+
+  Plan p;
+  plan_single(p, kind, dtype, (uint64_t)n, (uint32_t)num_k, o);
+  const size_t eb = (size_t)dtype_bytes(dtype);
+  // device output staging for host outputs, device k copy for host k lists
+  const size_t off_dout = p.total;
+  const size_t off_dk = align_up(off_dout + (host_out ? (size_t)num_k * eb : 0));
+  const size_t total = align_up(off_dk + (ksrc == KSource::HOST ? (size_t)num_k * 4 : 0));
+  if ((rc = ensure_arena(c, total))) return rc;
+  const size_t tbytes = p.table_bytes();
+  const size_t pin_k = align_up(tbytes);
+  const size_t pin_out = align_up(pin_k + (ksrc == KSource::HOST ? (size_t)num_k * 4 : 0));
+  const size_t pin_total = pin_out + (host_out ? (size_t)num_k * eb : 0);
+  if ((rc = ensure_pinned(c, pin_total))) return rc;
+
+  // plan tables -> pinned -> device (one copy)
+  p.write_tables(c->pinned);
+  HIP_TRY(hipMemcpyAsync(c->arena + p.off_tables, c->pinned, tbytes, hipMemcpyHostToDevice,
+                         c->stream));
+  Query* d_q = reinterpret_cast<Query*>(c->arena + p.off_q);
+  const uint32_t nk = (uint32_t)num_k;
+  const uint32_t qgrid = (nk + 255) / 256;
+  if (ksrc == KSource::TOPK) {
+    hipLaunchKernelGGL(k_topk_queries, dim3(qgrid), dim3(256), 0, c->stream, nk, (uint64_t)n, d_q);
+  } else {
+    const uint32_t* d_k = k;
+    if (ksrc == KSource::HOST) {
+      memcpy(c->pinned + pin_k, k, (size_t)num_k * 4);
+      HIP_TRY(hipMemcpyAsync(c->arena + off_dk, c->pinned + pin_k, (size_t)num_k * 4,
+                             hipMemcpyHostToDevice, c->stream));
+      d_k = reinterpret_cast<const uint32_t*>(c->arena + off_dk);
+    }
+    hipLaunchKernelGGL(k_queries, dim3(qgrid), dim3(256), 0, c->stream, d_k, nk, (uint64_t)n,
+                       d_q, c->status);
+  }
+  void* d_out = host_out ? (void*)(c->arena + off_dout) : out;
+  if ((rc = launch(c, dtype, p, d_vec, d_out))) return rc;
+  if (host_out) {
+    HIP_TRY(hipMemcpyAsync(c->pinned + pin_out, d_out, (size_t)num_k * eb, hipMemcpyDeviceToHost,
+                           c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    memcpy(out, c->pinned + pin_out, (size_t)num_k * eb);
+  }
+  return GGKS_OK;
+}
+
+}  // namespace ggks
+
+using namespace ggks;
+
+// ================================================================== C ABI
+extern "C" {
+
+const char* ggks_last_error(void) { return g_err.c_str(); }
+const char* ggks_version(void) { return "ggks-mi355x 0.1 (gfx950)"; }
+
+int ggks_ctx_create(ggks_ctx** ctx, int device, void* stream) {
+  if (!ctx) return fail(GGKS_EINVAL, "null ctx pointer");
+  ggks_ctx* c = new ggks_ctx();
+  int rc = ctx_init(c, device, (hipStream_t)stream);
+  if (rc) { delete c; return rc; }
+  *ctx = c;
+  return GGKS_OK;
+}
+
+int ggks_ctx_destroy(ggks_ctx* c) {
+  if (!c) return GGKS_OK;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  if (c->arena) (void)hipFree(c->arena);
+  if (c->pinned) (void)hipHostFree(c->pinned);
+  if (c->status) (void)hipFree(c->status);
+  delete c;
+  return GGKS_OK;
+}
+
+void* ggks_ctx_stream(ggks_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+size_t ggks_scratch_bytes(int kind, int dtype, int64_t n, int64_t num_k) {
+  if (dtype < 0 || dtype > 3 || n < 1 || num_k < 0) return 0;
+  Plan p;
+  ggks_opts o = {};
+  plan_single(p, kind, dtype, (uint64_t)n, (uint32_t)std::max<int64_t>(num_k, 1), o);
+  return p.total + align_up((size_t)num_k * (dtype_bytes(dtype) + 4));
+}
+
+int ggks_ctx_reserve(ggks_ctx* c, int kind, int dtype, int64_t n, int64_t num_k) {
+  if (!c) return fail(GGKS_EINVAL, "null ctx");
+  std::lock_guard<std::mutex> lk(c->mu);
+  HIP_TRY(hipSetDevice(c->device));
+  int rc = ensure_arena(c, ggks_scratch_bytes(kind, dtype, n, num_k));
+  if (rc) return rc;
+  Plan p;
+  ggks_opts o = {};
+  plan_single(p, kind, dtype, (uint64_t)n, (uint32_t)std::max<int64_t>(num_k, 1), o);
+  return ensure_pinned(c, p.table_bytes() + 4096);
+}
+
+int ggks_ctx_check(ggks_ctx* c) {
+  if (!c) return fail(GGKS_EINVAL, "null ctx");
+  uint32_t st = 0;
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipMemcpyAsync(&st, c->status, 4, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (st) {
+    HIP_TRY(hipMemsetAsync(c->status, 0, 4, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (st & STATUS_BAD_K) return fail(GGKS_EINVAL, "a device k list held k outside [1, n]");
+    return fail(GGKS_EDEVICE, "internal status 0x%x", st);
+  }
+  return GGKS_OK;
+}
+
+#define GGKS_CTX_OR_DEFAULT(c)                 \
+  do {                                         \
+    if (!(c)) {                                \
+      int rc_ = default_ctx(&(c));             \
+      if (rc_) return rc_;                     \
+    }                                          \
+  } while (0)
+
+int ggks_bucket_multiselect(ggks_ctx* c, int dtype, const void* d_vec, int64_t n,
+                            const uint32_t* h_k, int64_t num_k, void* h_out,
+                            const ggks_opts* opt) {
+  GGKS_CTX_OR_DEFAULT(c);
+  return select_one(c, GGKS_KIND_BUCKET, dtype, d_vec, n, h_k, num_k, KSource::HOST, h_out, true,
+                    opt);
+}
+int ggks_radix_multiselect(ggks_ctx* c, int dtype, const void* d_vec, int64_t n,
+                           const uint32_t* h_k, int64_t num_k, void* h_out,
+                           const ggks_opts* opt) {
+  GGKS_CTX_OR_DEFAULT(c);
+  return select_one(c, GGKS_KIND_RADIX, dtype, d_vec, n, h_k, num_k, KSource::HOST, h_out, true,
+                    opt);
+}
+int ggks_topk(ggks_ctx* c, int dtype, const void* d_vec, int64_t n, int64_t K, void* h_out,
+              const ggks_opts* opt) {
+  GGKS_CTX_OR_DEFAULT(c);
+  return select_one(c, GGKS_KIND_TOPK, dtype, d_vec, n, nullptr, K, KSource::TOPK, h_out, true,
+                    opt);
+}
+int ggks_bucket_multiselect_async(ggks_ctx* c, int dtype, const void* d_vec, int64_t n,
+                                  const uint32_t* d_k, int64_t num_k, void* d_out,
+                                  const ggks_opts* opt) {
+  GGKS_CTX_OR_DEFAULT(c);
+  return select_one(c, GGKS_KIND_BUCKET, dtype, d_vec, n, d_k, num_k, KSource::DEVICE, d_out,
+                    false, opt);
+}
+int ggks_radix_multiselect_async(ggks_ctx* c, int dtype, const void* d_vec, int64_t n,
+                                 const uint32_t* d_k, int64_t num_k, void* d_out,
+                                 const ggks_opts* opt) {
+  GGKS_CTX_OR_DEFAULT(c);
+  return select_one(c, GGKS_KIND_RADIX, dtype, d_vec, n, d_k, num_k, KSource::DEVICE, d_out,
+                    false, opt);
+}
+int ggks_topk_async(ggks_ctx* c, int dtype, const void* d_vec, int64_t n, int64_t K, void* d_out,
+                    const ggks_opts* opt) {
+  GGKS_CTX_OR_DEFAULT(c);
+  return select_one(c, GGKS_KIND_TOPK, dtype, d_vec, n, nullptr, K, KSource::TOPK, d_out, false,
+                    opt);
+}
+
+#define GGKS_TYPED(NAME, T, DT)                                                          \
+  int NAME(const T* d_vec, int64_t n, const uint32_t* h_k, int32_t num_k, T* h_out,      \
+           const ggks_opts* opt) {                                                       \
+    return ggks_bucket_multiselect(nullptr, DT, d_vec, n, h_k, num_k, h_out, opt);       \
+  }
+GGKS_TYPED(ggks_bucket_multiselect_f32, float, GGKS_F32)
+GGKS_TYPED(ggks_bucket_multiselect_f64, double, GGKS_F64)
+GGKS_TYPED(ggks_bucket_multiselect_u32, uint32_t, GGKS_U32)
+GGKS_TYPED(ggks_bucket_multiselect_i32, int32_t, GGKS_I32)
+
+// ---------------------------------------------------------- plumbing
+int ggks_dev_alloc(void** ptr, size_t bytes) {
+  if (!ptr) return fail(GGKS_EINVAL, "null pointer");
+  if (hipMalloc(ptr, bytes ? bytes : 1) != hipSuccess) {
+    (void)hipGetLastError();
+    *ptr = nullptr;
+    return fail(GGKS_ENOMEM, "hipMalloc(%zu) failed", bytes);
+  }
+  return GGKS_OK;
+}
+int ggks_dev_free(void* ptr) {
+  HIP_TRY(hipFree(ptr));
+  return GGKS_OK;
+}
+int ggks_h2d(void* dst, const void* src, size_t bytes) {
+  HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return GGKS_OK;
+}
+int ggks_d2h(void* dst, const void* src, size_t bytes) {
+  HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return GGKS_OK;
+}
+int ggks_d2d(void* dst, const void* src, size_t bytes) {
+  HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+  return GGKS_OK;
+}
+int ggks_sync(void) {
+  HIP_TRY(hipDeviceSynchronize());
+  return GGKS_OK;
+}
+int ggks_device_count(int* count) {
+  HIP_TRY(hipGetDeviceCount(count));
+  return GGKS_OK;
+}
+int ggks_set_device(int device) {
+  HIP_TRY(hipSetDevice(device));
+  return GGKS_OK;
+}
+int ggks_dev_fill(void* ptr, int value, size_t bytes) {
+  HIP_TRY(hipMemset(ptr, value, bytes));
+  return GGKS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,97 @@
+// ggks_types.hpp -- device-side tables shared by the selection kernels.
+//
+// The engine works on a table of top-level segments (one for a single-vector
+// select, thousands for a batch).  A segment is split into chunks; every chunk
+// is one work-group in the two streaming passes (count, compact), so the
+// per-chunk histogram rows written by the count pass give every work-group its
+// private write offsets in the compaction pass.  Each bucket that holds a
+// requested rank becomes a child; children are finished in LDS.
+#pragma once
+#include <stdint.h>
+
+namespace ggks {
+
+constexpr int kThreads = 1024;             // work-group size of every kernel
+constexpr uint32_t kMaxBuckets = 32768;    // histogram bins per segment (LDS)
+constexpr uint32_t kMaxPrefixBits = 12;    // LUT prefixes per segment <= 4096
+constexpr uint32_t kCursorCap = 8192;      // LDS write cursors in compaction
+constexpr uint32_t kFinishLdsBytes = 65536;// keys kept in LDS by the finisher
+constexpr uint32_t kQCap = 1024;           // queries per finisher batch
+constexpr uint32_t kGroupMax = 32;         // prefix groups per histogram pass
+constexpr uint32_t kReduceBins = 64;       // bins per reduce work item
+constexpr uint32_t kSortRanks = 32;        // finisher sorts a child above this
+
+// Bucket-function modes of a segment.
+enum SegMode : uint32_t {
+  MODE_SAMPLE = 0,  // prefix LUT sized from a random sample of the segment
+  MODE_RADIX = 1,   // the top radix_bits bits of the key (MSD radix digit)
+  MODE_RANGE = 2    // equal-width buckets over the known key range [lo, hi]
+};
+
+// A requested order statistic: 1-based ascending rank inside its segment or
+// child, and the index of the output slot it fills.
+struct Query {
+  uint32_t rank;
+  uint32_t out;
+};
+
+struct SegDesc {          // planned on the host, read-only on the device
+  uint64_t data_off;      // element offset of the segment in the source
+  uint64_t len;           // elements
+  uint64_t row_base;      // u32 offset of the first chunk's histogram row
+  uint64_t red_base;      // element offset of its children in the reduced buffer
+  uint64_t lo, hi;        // MODE_RANGE: key range of every element
+  uint32_t bmax;          // bucket budget including the two tail buckets
+  uint32_t pbits;         // LUT prefix bits (P <= 2^pbits)
+  uint32_t nchunks;
+  uint32_t sample_off;    // first sample slot (key units)
+  uint32_t nsample;
+  uint32_t lut_off;       // first LUT entry (u32)
+  uint32_t tot_off;       // first total (u32), bmax entries
+  uint32_t bits_off;      // first bitmap word (u32), bmax/32 words
+  uint32_t q_off, q_cnt;  // its queries; child slots use the same range
+  uint32_t mode;          // SegMode
+  uint32_t radix_bits;    // MODE_RADIX digit width
+};
+
+struct SegState {         // written on the device by the LUT kernel
+  uint64_t kb;            // smallest key of the interior buckets
+  uint64_t kmax;          // largest sampled key
+  uint32_t s;             // prefix shift: prefix = (key - kb) >> s
+  uint32_t P;             // number of prefixes
+  uint32_t B;             // buckets including tails
+  uint32_t Bi;            // interior buckets
+  uint32_t nact;          // active buckets (children)
+  uint32_t pad;
+};
+
+struct Chunk {            // one work-group of the streaming passes
+  uint32_t seg;
+  uint32_t idx;           // chunk index inside the segment (row index)
+  uint64_t start;         // element offset inside the segment
+  uint64_t len;
+};
+
+struct ReduceItem {       // kReduceBins consecutive bins of one segment
+  uint32_t seg;
+  uint32_t group;
+};
+
+struct Child {            // one bucket to finish, or one small whole segment
+  uint64_t start;         // element offset in its source buffer
+  uint64_t len;
+  uint64_t lo, hi;        // key range of its elements (unless need_range)
+  uint32_t q_off, q_cnt;  // its queries (ranks inside the child)
+  uint32_t src;           // 0: the input buffer, 1: the reduced buffer
+  uint32_t need_range;    // 1: lo/hi unknown, computed by the finisher
+  uint32_t bucket;        // bucket of the parent segment it came from
+  uint32_t seg;           // parent segment
+};
+
+// Per-context status bits written by kernels.
+enum StatusBits : uint32_t {
+  STATUS_BAD_K = 1u,      // a device k list held k outside [1, n]
+  STATUS_OVERFLOW = 2u    // internal capacity exceeded (should not happen)
+};
+
+}  // namespace ggks

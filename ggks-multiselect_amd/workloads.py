@@ -1,0 +1,131 @@
+"""Seeded synthetic inputs for the benchmark configurations and the tests.
+
+The reference draws its vectors from cuRAND streams seeded by the clock
+(generateProblems.cu:25-428, compareMultiselect.hpp:106-126), which cannot be
+reproduced; these generators keep its distribution shapes with a counter-based
+SplitMix64 stream, so every input is a pure function of (seed, index) and the
+same on every machine.  -0.0 is canonicalised to +0.0 and nothing is NaN.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+GOLDEN = np.uint64(0x9E3779B97F4A7C15)
+M1 = np.uint64(0xBF58476D1CE4E5B9)
+M2 = np.uint64(0x94D049BB133111EB)
+CHUNK = 1 << 24
+
+
+def splitmix64(seed: int, start: int, count: int) -> np.ndarray:
+    """Outputs start..start+count-1 of the SplitMix64 sequence of `seed`."""
+    with np.errstate(over="ignore"):
+        i = np.arange(start + 1, start + count + 1, dtype=np.uint64)
+        z = np.uint64(seed & 0xFFFFFFFFFFFFFFFF) + i * GOLDEN
+        z = (z ^ (z >> np.uint64(30))) * M1
+        z = (z ^ (z >> np.uint64(27))) * M2
+        return z ^ (z >> np.uint64(31))
+
+
+def _chunked(n: int, fn, dtype) -> np.ndarray:
+    out = np.empty(n, dtype)
+    for s in range(0, n, CHUNK):
+        e = min(n, s + CHUNK)
+        out[s:e] = fn(s, e - s)
+    return out
+
+
+def uniform_f32(seed: int, n: int) -> np.ndarray:
+    """U[0,1) on the 24-bit grid: (x >> 40) * 2^-24."""
+    return _chunked(n, lambda s, c: (splitmix64(seed, s, c) >> np.uint64(40)).astype(np.float32)
+                    * np.float32(2.0 ** -24), np.float32)
+
+
+def uniform_f64(seed: int, n: int) -> np.ndarray:
+    """U[0,1) on the 53-bit grid."""
+    return _chunked(n, lambda s, c: (splitmix64(seed, s, c) >> np.uint64(11)).astype(np.float64)
+                    * (2.0 ** -53), np.float64)
+
+
+def _normal64(seed: int, s: int, c: int) -> np.ndarray:
+    # Box-Muller in double from two independent streams of the same seed
+    u1 = ((splitmix64(seed, s, c) >> np.uint64(11)).astype(np.float64) + 1.0) * (2.0 ** -53)
+    u2 = (splitmix64(seed ^ 0x5DEECE66D, s, c) >> np.uint64(11)).astype(np.float64) * (2.0 ** -53)
+    return np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2)
+
+
+def _canon(x: np.ndarray) -> np.ndarray:
+    x[x == 0] = 0  # -0.0 -> +0.0
+    return x
+
+
+def normal_f32(seed: int, n: int) -> np.ndarray:
+    return _canon(_chunked(n, lambda s, c: _normal64(seed, s, c).astype(np.float32), np.float32))
+
+
+def normal_f64(seed: int, n: int) -> np.ndarray:
+    return _canon(_chunked(n, lambda s, c: _normal64(seed, s, c), np.float64))
+
+
+def cauchy_f32(seed: int, n: int) -> np.ndarray:
+    """Cauchy(0,1) = tan(pi (u - 1/2)), clipped to finite float32
+    (generateProblems.cu:212-247 shape)."""
+    fmax = np.finfo(np.float32).max
+
+    def gen(s, c):
+        u = (splitmix64(seed, s, c) >> np.uint64(11)).astype(np.float64) * (2.0 ** -53)
+        u = np.clip(u, 2.0 ** -40, 1.0 - 2.0 ** -40)
+        return np.clip(np.tan(np.pi * (u - 0.5)), -fmax, fmax).astype(np.float32)
+    return _canon(_chunked(n, gen, np.float32))
+
+
+def uint_range(seed: int, n: int, lo: int, hi: int) -> np.ndarray:
+    """uint32 uniform in [lo, hi] (inclusive)."""
+    span = np.uint64(hi - lo + 1)
+    return _chunked(n, lambda s, c: (np.uint64(lo) + (splitmix64(seed, s, c) >> np.uint64(16)) % span)
+                    .astype(np.uint32), np.uint32)
+
+
+def int_range(seed: int, n: int, lo: int, hi: int) -> np.ndarray:
+    span = hi - lo + 1
+    return _chunked(n, lambda s, c: (np.int64(lo) + ((splitmix64(seed, s, c) >> np.uint64(16))
+                                                     % np.uint64(span)).astype(np.int64))
+                    .astype(np.int32), np.int32)
+
+
+def plateau_f32(seed: int, n: int, value: float = 0.5, frac: float = 0.5) -> np.ndarray:
+    """Exactly round(frac*n) entries equal to `value`, the rest U[0,1), in a
+    seeded random order."""
+    x = uniform_f32(seed, n)
+    m = int(round(frac * n))
+    # positions: a seeded permutation's first m slots
+    keys = splitmix64(seed ^ 0xA5A5A5A5, 0, n)
+    idx = np.argpartition(keys, m - 1)[:m] if m > 0 else np.empty(0, np.int64)
+    x[idx] = np.float32(value)
+    return x
+
+
+def permutation_u32(seed: int, n: int) -> np.ndarray:
+    """A seeded permutation of 1..n (the k-th largest is n - k + 1)."""
+    keys = splitmix64(seed, 0, n)
+    return (np.argsort(keys, kind="stable") + 1).astype(np.uint32)
+
+
+# ---------------------------------------------------------------- k sets
+def k_percentiles(n: int) -> np.ndarray:
+    """101 ranks k = round(p (n-1)) + 1, p = 0.00 .. 1.00 (p = 0 is the max)."""
+    p = np.arange(101, dtype=np.float64) / 100.0
+    return (np.rint(p * (n - 1)) + 1).astype(np.uint32)
+
+
+def k_uniform(seed: int, n: int, count: int, dedup: bool = True) -> np.ndarray:
+    """count ranks uniform in [1, n]; sorted and deduplicated when dedup."""
+    k = (splitmix64(seed, 0, count) % np.uint64(n) + np.uint64(1)).astype(np.uint32)
+    return np.unique(k) if dedup else k
+
+
+def k_cluster(seed: int, n: int, count: int = 256, width: int = 1000) -> np.ndarray:
+    """count distinct ranks in [n/2 - width/2, n/2 + width/2 - 1] plus 1 and n."""
+    lo = n // 2 - width // 2
+    keys = splitmix64(seed, 0, width)
+    pick = np.sort(np.argsort(keys, kind="stable")[:count]) + lo
+    return np.concatenate([pick.astype(np.uint32), np.array([1, n], np.uint32)])

@@ -1,0 +1,151 @@
+/*
+ * ggks.h -- C ABI of the MI355X (gfx950) selection library libggks.so.
+ *
+ * Order statistics on device-resident vectors: bucket multiselect, MSD radix
+ * multiselect and top-K, for float32, float64, uint32 and int32.
+ *
+ * Conventions shared by every entry point (they follow the reference):
+ *  - k values are 1-based ranks of the k-th LARGEST element: k = 1 is the
+ *    maximum, k = n the minimum (bucketMultiselect.cu:845-846,
+ *    multiselectTimingFunctions.cu:41-48).  k lists may be unsorted and may
+ *    repeat; outputs follow the caller's k order (bucketMultiselect.cu:615-628,
+ *    :313-325).
+ *  - Top-K returns the K largest values in descending order
+ *    (multiselectTimingFunctions.cu:155-177).
+ *  - Order is the order of the order-preserving unsigned key conversion
+ *    (keyConversion.cu:8-42), i.e. the reference's radix order: IEEE order
+ *    except that -0.0 sorts before +0.0.  NaN inputs are outside the contract.
+ *  - The input vector is never modified (the reference's radix paths overwrite
+ *    it, merrillSelect.cu:210-259; opts->destructive is accepted and ignored).
+ *  - Return 0 on success or a negative GGKS_E* code; outputs are untouched on
+ *    error.  ggks_last_error() gives the message of the calling thread's last
+ *    failure.
+ *  - n must satisfy 1 <= n < 2^32 (the reference uses 32-bit lengths).
+ *
+ * The scratch the kernels need comes from a per-context arena that grows on
+ * demand and is reused across calls; ggks_scratch_bytes() reports the size
+ * for admission control.  Calls on one context are serialised by a mutex.
+ */
+#ifndef GGKS_H
+#define GGKS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum ggks_dtype { GGKS_F32 = 0, GGKS_F64 = 1, GGKS_U32 = 2, GGKS_I32 = 3 };
+
+enum ggks_status {
+  GGKS_OK = 0,
+  GGKS_EINVAL = -1,      /* bad argument: n < 1, k outside [1, n], null ptr */
+  GGKS_ENOMEM = -2,      /* device or pinned memory could not be allocated */
+  GGKS_EDEVICE = -3,     /* a HIP call failed (see ggks_last_error) */
+  GGKS_EUNSUPPORTED = -4 /* argument combination not supported */
+};
+
+enum ggks_kind {
+  GGKS_KIND_BUCKET = 0, /* bucket multiselect */
+  GGKS_KIND_RADIX = 1,  /* MSD radix multiselect */
+  GGKS_KIND_TOPK = 2    /* top-K (descending) */
+};
+
+/* Options; a NULL pointer means all defaults (zeros). */
+typedef struct ggks_opts {
+  int32_t blocks;       /* chunk count hint for the streaming passes; 0 = tuned */
+  int32_t threads;      /* accepted for interface parity (reference passes the
+                           device's maxThreadsPerBlock); kernels use 1024 */
+  int32_t buckets_log2; /* bucket budget as log2 (8..15); 0 = tuned */
+  int32_t destructive;  /* accepted; the input is never modified */
+  uint64_t seed;        /* seed of the splitter sample; 0 = fixed default */
+  int32_t radix_bits;   /* radix path: digit width per level (4..15); 0 = 15 */
+  int32_t reserved[5];
+} ggks_opts;
+
+typedef struct ggks_ctx ggks_ctx;
+
+/* ---- contexts ------------------------------------------------------ */
+/* stream: a hipStream_t (NULL = the device's null stream). */
+int ggks_ctx_create(ggks_ctx **ctx, int device, void *stream);
+int ggks_ctx_destroy(ggks_ctx *ctx);
+/* Bytes of device scratch one call of `kind` needs (admission control). */
+size_t ggks_scratch_bytes(int kind, int dtype, int64_t n, int64_t num_k);
+const char *ggks_last_error(void);
+const char *ggks_version(void);
+
+/* ---- single vector, host k list / host outputs ----------------------
+ * Replaces BucketMultiselect::bucketMultiselectWrapper
+ * (bucketMultiselect.cu:837-871): device vector, length, host k array,
+ * host output array.  ctx may be NULL (per-device default context, current
+ * device, null stream). */
+int ggks_bucket_multiselect(ggks_ctx *ctx, int dtype, const void *d_vec,
+                            int64_t n, const uint32_t *h_k, int64_t num_k,
+                            void *h_out, const ggks_opts *opt);
+int ggks_bucket_multiselect_f32(const float *d_vec, int64_t n,
+                                const uint32_t *h_k, int32_t num_k,
+                                float *h_out, const ggks_opts *opt);
+int ggks_bucket_multiselect_f64(const double *d_vec, int64_t n,
+                                const uint32_t *h_k, int32_t num_k,
+                                double *h_out, const ggks_opts *opt);
+int ggks_bucket_multiselect_u32(const uint32_t *d_vec, int64_t n,
+                                const uint32_t *h_k, int32_t num_k,
+                                uint32_t *h_out, const ggks_opts *opt);
+int ggks_bucket_multiselect_i32(const int32_t *d_vec, int64_t n,
+                                const uint32_t *h_k, int32_t num_k,
+                                int32_t *h_out, const ggks_opts *opt);
+
+/* Replaces RadixSelect::RadixSelectWrapper (merrillSelect.cu:210-259) and
+ * InplaceRadix::inplaceRadixSelectWrapper (inplaceRadixSelect.cu:83-108),
+ * generalised to many k: MSD digits of the converted key, no sampling. */
+int ggks_radix_multiselect(ggks_ctx *ctx, int dtype, const void *d_vec,
+                           int64_t n, const uint32_t *h_k, int64_t num_k,
+                           void *h_out, const ggks_opts *opt);
+
+/* Replaces randomizedTopkSelectWrapper (randomizedTopkSelect.cu:210-213) and
+ * timeBucketTopkselect (multiselectTimingFunctions.cu:204-231): the K largest
+ * values, descending, into the caller's host buffer h_out[K]. */
+int ggks_topk(ggks_ctx *ctx, int dtype, const void *d_vec, int64_t n,
+              int64_t K, void *h_out, const ggks_opts *opt);
+
+/* ---- stream-ordered variants: device k list and device outputs ------
+ * Nothing is synchronised and no host memory is touched, so these can be
+ * captured in a hipGraph once the context's arena is large enough (call
+ * ggks_ctx_reserve first).  An invalid k is reported by
+ * ggks_ctx_check() after the stream has drained. */
+int ggks_bucket_multiselect_async(ggks_ctx *ctx, int dtype, const void *d_vec,
+                                  int64_t n, const uint32_t *d_k,
+                                  int64_t num_k, void *d_out,
+                                  const ggks_opts *opt);
+int ggks_radix_multiselect_async(ggks_ctx *ctx, int dtype, const void *d_vec,
+                                 int64_t n, const uint32_t *d_k, int64_t num_k,
+                                 void *d_out, const ggks_opts *opt);
+int ggks_topk_async(ggks_ctx *ctx, int dtype, const void *d_vec, int64_t n,
+                    int64_t K, void *d_out, const ggks_opts *opt);
+/* Grow the arena so that a call of `kind` with these sizes allocates
+ * nothing. */
+int ggks_ctx_reserve(ggks_ctx *ctx, int kind, int dtype, int64_t n,
+                     int64_t num_k);
+/* After the stream drained: GGKS_OK, or GGKS_EINVAL if a device k list held
+ * a k outside [1, n] since the last check (the flag is then cleared). */
+int ggks_ctx_check(ggks_ctx *ctx);
+void *ggks_ctx_stream(ggks_ctx *ctx);
+
+/* ---- device plumbing for hosts without a GPU framework --------------
+ * Thin wrappers over the HIP runtime this library is linked with. */
+int ggks_dev_alloc(void **ptr, size_t bytes);
+int ggks_dev_free(void *ptr);
+int ggks_h2d(void *dst, const void *src, size_t bytes);
+int ggks_d2h(void *dst, const void *src, size_t bytes);
+int ggks_d2d(void *dst, const void *src, size_t bytes);
+int ggks_sync(void);
+int ggks_device_count(int *count);
+int ggks_set_device(int device);
+/* Fill a device buffer with a byte pattern (L2/MALL flush helper). */
+int ggks_dev_fill(void *ptr, int value, size_t bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GGKS_H */
