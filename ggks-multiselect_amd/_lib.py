@@ -25,8 +25,11 @@ EXPORTS = [
     "ggks_bucket_multiselect_async", "ggks_radix_multiselect_async", "ggks_topk_async",
     "ggks_ctx_reserve", "ggks_ctx_check", "ggks_ctx_stream", "ggks_dev_alloc",
     "ggks_dev_free", "ggks_h2d", "ggks_d2h", "ggks_d2d", "ggks_sync",
-    "ggks_device_count", "ggks_set_device", "ggks_dev_fill",
+    "ggks_device_count", "ggks_set_device", "ggks_dev_fill", "ggks_ctx_profile_arm",
+    "ggks_ctx_profile_read",
 ]
+NSTAGES = 8
+STAGES = ["queries", "sample", "lut", "count", "reduce", "findk", "compact", "finish"]
 
 
 class GGKSError(RuntimeError):
@@ -81,6 +84,8 @@ def lib() -> ctypes.CDLL:
         "ggks_device_count": ([ctypes.POINTER(i32)], i32),
         "ggks_set_device": ([i32], i32),
         "ggks_dev_fill": ([vp, i32, sz], i32),
+        "ggks_ctx_profile_arm": ([vp, i32], i32),
+        "ggks_ctx_profile_read": ([vp, vp, i32], i32),
     }
     for t in ("f32", "f64", "u32", "i32"):
         sig[f"ggks_bucket_multiselect_{t}"] = ([vp, i64, vp, ctypes.c_int32, vp, optp], i32)

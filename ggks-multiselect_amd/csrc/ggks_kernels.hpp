@@ -85,6 +85,22 @@ __device__ uint32_t block_scan_array(uint32_t* a, uint32_t L, uint32_t* wsum) {
   return total;
 }
 
+// f(i, p[i]) for i = threadIdx.x, +kThreads, ... < n with 16 loads in flight
+// per thread (the loops that read another kernel's output are latency bound).
+template <typename T, typename F>
+__device__ __forceinline__ void strided_for(const T* __restrict__ p, uint32_t n, F&& f) {
+  constexpr uint32_t D = 16;
+  uint32_t i = threadIdx.x;
+  for (; i + (D - 1) * kThreads < n; i += D * kThreads) {
+    T v[D];
+#pragma unroll
+    for (uint32_t u = 0; u < D; ++u) v[u] = p[i + u * kThreads];
+#pragma unroll
+    for (uint32_t u = 0; u < D; ++u) f(i + u * kThreads, v[u]);
+  }
+  for (; i < n; i += kThreads) f(i, p[i]);
+}
+
 template <typename KT>
 __device__ __forceinline__ uint32_t mulhi_nb(KT x, uint32_t nb);
 template <>
@@ -119,14 +135,18 @@ struct Bucketer {
     smask = st.s >= W ? KeyBits<KT>::MAX : ((KT(1) << st.s) - KT(1));
     lsh = st.s == 0 ? 0u : (W - st.s);
   }
+  // Branch-free: the two tail cases are selected after the look-up, so a
+  // wavefront issues all its LDS reads without exec-mask changes.
   __device__ __forceinline__ uint32_t operator()(KT key, const uint32_t* lut) const {
-    if (key < kb) return 0u;
-    KT low = key - kb;
-    KT p = (low >> s1) >> s2;
-    if (p >= (KT)P) return Bm1;
-    uint32_t e = lut[(uint32_t)p];
-    uint32_t j = mulhi_nb<KT>((low & smask) << lsh, e >> 16);
-    return 1u + (e & 0xFFFFu) + j;
+    const bool below = key < kb;
+    const KT low = key - kb;
+    const KT p = (low >> s1) >> s2;
+    const bool above = p >= (KT)P;
+    const uint32_t pi = (below | above) ? 0u : (uint32_t)p;
+    const uint32_t e = lut[pi];
+    const uint32_t j = mulhi_nb<KT>((low & smask) << lsh, e >> 16);
+    const uint32_t b = 1u + (e & 0xFFFFu) + j;
+    return below ? 0u : (above ? Bm1 : b);
   }
 };
 
@@ -282,11 +302,10 @@ __global__ __launch_bounds__(kThreads) void k_lut(const SegDesc* __restrict__ se
 
   if (sd.mode == MODE_SAMPLE) {
     KT lo = KeyBits<KT>::MAX, hi = 0;
-    for (uint32_t i = tid; i < sd.nsample; i += kThreads) {
-      KT k = samples[sd.sample_off + i];
+    strided_for(samples + sd.sample_off, sd.nsample, [&](uint32_t, KT k) {
       lo = k < lo ? k : lo;
       hi = k > hi ? k : hi;
-    }
+    });
     for (int d = 32; d >= 1; d >>= 1) {
       KT a = __shfl_xor(lo, d, 64), b = __shfl_xor(hi, d, 64);
       lo = a < lo ? a : lo;
@@ -337,11 +356,9 @@ __global__ __launch_bounds__(kThreads) void k_lut(const SegDesc* __restrict__ se
   } else {
     for (uint32_t p = tid; p < P; p += kThreads) cnt[p] = 0;
     __syncthreads();
-    for (uint32_t i = tid; i < sd.nsample; i += kThreads) {
-      KT k = samples[sd.sample_off + i];
-      uint32_t p = (uint32_t)((k - kmin) >> s);
-      atomicAdd(&cnt[p], 1u);
-    }
+    strided_for(samples + sd.sample_off, sd.nsample, [&](uint32_t, KT k) {
+      atomicAdd(&cnt[(uint32_t)((k - kmin) >> s)], 1u);
+    });
     __syncthreads();
     // prefixes holding no sample: a run of them shares one bucket
     uint32_t nonempty = 0, runs = 0;
@@ -418,61 +435,85 @@ __global__ __launch_bounds__(kThreads) void k_count(const KT* __restrict__ src,
 }
 
 // Sum the chunk rows of kReduceBins buckets; rewrite each row entry as the
-// chunk's exclusive offset inside its bucket.  256 threads: 4 chunk quarters.
-__global__ __launch_bounds__(256) void k_reduce(const SegDesc* __restrict__ segs,
-                                                const SegState* __restrict__ states,
-                                                const ReduceItem* __restrict__ items,
-                                                uint32_t* __restrict__ rows,
-                                                uint32_t* __restrict__ totals) {
-  __shared__ uint32_t part[4][kReduceBins];
+// chunk's exclusive offset inside its bucket.  1024 threads = 16 groups of
+// chunks x 64 bins; each thread keeps its (at most kReduceRegs) row entries in
+// registers between the sum and the rewrite.
+constexpr uint32_t kReduceParts = kThreads / kReduceBins;  // 16
+constexpr uint32_t kReduceRegs = kMaxChunks / kReduceParts;  // chunks per part
+
+__global__ __launch_bounds__(kThreads) void k_reduce(const SegDesc* __restrict__ segs,
+                                                     const SegState* __restrict__ states,
+                                                     const ReduceItem* __restrict__ items,
+                                                     uint32_t* __restrict__ rows,
+                                                     uint32_t* __restrict__ totals) {
+  __shared__ uint32_t part[kReduceParts][kReduceBins];
   const ReduceItem it = items[blockIdx.x];
   const SegDesc& sd = segs[it.seg];
   const uint32_t B = states[it.seg].B;
-  const uint32_t lane = threadIdx.x & (kReduceBins - 1), q = threadIdx.x / kReduceBins;
+  const uint32_t lane = threadIdx.x % kReduceBins, q = threadIdx.x / kReduceBins;
   const uint32_t bin = it.group * kReduceBins + lane;
   const bool valid = bin < B;
-  const uint32_t nch = sd.nchunks;
-  const uint32_t c0 = (uint32_t)((uint64_t)nch * q / 4), c1 = (uint32_t)((uint64_t)nch * (q + 1) / 4);
+  const uint32_t nch = sd.nchunks;  // <= kMaxChunks (planner)
+  const uint32_t c0 = (uint32_t)((uint64_t)nch * q / kReduceParts);
+  const uint32_t c1 = (uint32_t)((uint64_t)nch * (q + 1) / kReduceParts);
   uint32_t* base = rows + sd.row_base + bin;
   const uint64_t stride = sd.bmax;
+  uint32_t v[kReduceRegs];
   uint32_t sum = 0;
-  if (valid) {
-    uint32_t c = c0;
-    for (; c + 8 <= c1; c += 8) {
-      uint32_t v[8];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) v[u] = base[(uint64_t)(c + u) * stride];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) sum += v[u];
-    }
-    for (; c < c1; ++c) sum += base[(uint64_t)c * stride];
+  for (uint32_t u = 0; u < kReduceRegs; ++u) {
+    v[u] = (valid && c0 + u < c1) ? base[(uint64_t)(c0 + u) * stride] : 0u;
+    sum += v[u];
   }
   part[q][lane] = sum;
   __syncthreads();
   uint32_t off = 0, total = 0;
-  for (uint32_t j = 0; j < 4; ++j) {
-    uint32_t v = part[j][lane];
-    if (j < q) off += v;
-    total += v;
+#pragma unroll
+  for (uint32_t j = 0; j < kReduceParts; ++j) {
+    uint32_t w = part[j][lane];
+    off += j < q ? w : 0u;
+    total += w;
   }
-  if (valid) {
-    uint32_t run = off;
-    for (uint32_t c = c0; c < c1; ++c) {
-      uint32_t v = base[(uint64_t)c * stride];
-      base[(uint64_t)c * stride] = run;
-      run += v;
-    }
-    if (q == 0) totals[sd.tot_off + bin] = total;
+  uint32_t run = off;
+#pragma unroll
+  for (uint32_t u = 0; u < kReduceRegs; ++u) {
+    if (valid && c0 + u < c1) base[(uint64_t)(c0 + u) * stride] = run;
+    run += v[u];
   }
+  if (valid && q == 0) totals[sd.tot_off + bin] = total;
 }
 
 // Per segment: ranks -> buckets, children, and the ranks grouped by child.
+// The i-th query of a segment: explicit, from a raw k list (k-th largest ->
+// rank len - k + 1, output slot i; a bad k raises a status bit and is clamped
+// to k = 1), or rank len - i for a top-K.
+__device__ __forceinline__ Query seg_query(const SegDesc& sd, const Query* __restrict__ queries,
+                                           const uint32_t* __restrict__ kraw, uint32_t i,
+                                           uint32_t* __restrict__ status) {
+  Query q;
+  if (sd.qsrc == QSRC_QUERIES) return queries[sd.q_off + i];
+  if (sd.qsrc == QSRC_K) {
+    uint32_t k = kraw[sd.q_off + i];
+    if (k < 1 || (uint64_t)k > sd.len) {
+      atomicOr(status, (uint32_t)STATUS_BAD_K);
+      k = 1;
+    }
+    q.rank = (uint32_t)(sd.len - k + 1);
+  } else {
+    q.rank = (uint32_t)(sd.len - i);
+  }
+  q.out = sd.out_off + i;
+  return q;
+}
+
 template <typename KT, int C>
 __global__ __launch_bounds__(kThreads) void k_findk(const SegDesc* __restrict__ segs,
                                                     SegState* __restrict__ states,
                                                     const uint32_t* __restrict__ lut_pool,
                                                     const uint32_t* __restrict__ totals,
                                                     const Query* __restrict__ queries,
+                                                    const uint32_t* __restrict__ kraw,
+                                                    uint32_t* __restrict__ status,
                                                     uint32_t* __restrict__ qb,
                                                     uint32_t* __restrict__ qr,
                                                     Child* __restrict__ children,
@@ -493,8 +534,8 @@ __global__ __launch_bounds__(kThreads) void k_findk(const SegDesc* __restrict__ 
   uint32_t* wb = bits + sd.bmax / 32;            // bmax / 32
   uint32_t* wsum = wb + sd.bmax / 32;            // 17 (+ pad)
   const uint32_t tid = threadIdx.x;
-  for (uint32_t i = tid; i < B; i += kThreads) hist[i] = totals[sd.tot_off + i];
-  for (uint32_t i = tid; i < st.P; i += kThreads) lut[i] = lut_pool[sd.lut_off + i];
+  strided_for(totals + sd.tot_off, B, [&](uint32_t i, uint32_t v) { hist[i] = v; });
+  strided_for(lut_pool + sd.lut_off, st.P, [&](uint32_t i, uint32_t v) { lut[i] = v; });
   for (uint32_t i = tid; i < nwords; i += kThreads) bits[i] = 0;
   __syncthreads();
   const uint32_t len = block_scan_array(hist, B, wsum);  // hist = exclusive
@@ -504,7 +545,7 @@ __global__ __launch_bounds__(kThreads) void k_findk(const SegDesc* __restrict__ 
   // A: bucket of every rank; single-key buckets are answered here
   const uint32_t NONE = 0xFFFFFFFFu;
   for (uint32_t i = tid; i < sd.q_cnt; i += kThreads) {
-    const Query q = queries[sd.q_off + i];
+    const Query q = seg_query(sd, queries, kraw, i, status);
     const uint32_t r1 = q.rank - 1;
     uint32_t lo = 0, hi = B - 1;  // largest b with hist[b] <= r1
     while (lo < hi) {
@@ -588,7 +629,7 @@ __global__ __launch_bounds__(kThreads) void k_findk(const SegDesc* __restrict__ 
     uint32_t pos = atomicAdd(&hist[u], 1u);
     Query q;
     q.rank = qr[sd.q_off + i];
-    q.out = queries[sd.q_off + i].out;
+    q.out = seg_query(sd, queries, kraw, i, status).out;
     cq[sd.q_off + pos] = q;
   }
   // unused child slots finish nothing
@@ -600,6 +641,20 @@ __global__ __launch_bounds__(kThreads) void k_findk(const SegDesc* __restrict__ 
 }
 
 // Pass 2: copy the keys of active buckets into their children's ranges.
+//
+// Every wavefront stages its active keys in LDS (slot = ballot prefix count,
+// no atomics) and flushes them in batches: one LDS cursor atomic per staged
+// key, 64 lanes at a time, then the global store.  The body runs in uniform
+// rounds so every lane of a wavefront takes part in every ballot.
+constexpr uint32_t kStageCap = 512;  // staged keys per wavefront (>= 2 groups of 256)
+constexpr uint32_t kWaves = kThreads / 64;
+
+template <typename KT>
+__host__ __device__ constexpr size_t compact_lds_bytes(uint32_t pbits, uint32_t bmax) {
+  return (size_t)(1u << pbits) * 4 + (size_t)((bmax + 31) / 32) * 8 + (size_t)kCursorCap * 4 +
+         (size_t)kWaves * kStageCap * (sizeof(KT) + 2);
+}
+
 template <typename KT, int C>
 __global__ __launch_bounds__(kThreads) void k_compact(const KT* __restrict__ src,
                                                       const SegDesc* __restrict__ segs,
@@ -611,26 +666,27 @@ __global__ __launch_bounds__(kThreads) void k_compact(const KT* __restrict__ src
                                                       const uint32_t* __restrict__ wbase_pool,
                                                       const Child* __restrict__ children,
                                                       KT* __restrict__ red) {
-  extern __shared__ uint32_t lds[];
+  extern __shared__ __align__(16) uint32_t lds[];
   const Chunk ch = chunks[blockIdx.x];
   const SegDesc& sd = segs[ch.seg];
   const SegState st = states[ch.seg];
   const uint32_t U = st.nact;
   if (U == 0) return;
   const uint32_t nwords = (st.B + 31) / 32;
-  uint32_t* lut = lds;
-  uint32_t* bits = lut + (1u << sd.pbits);
-  uint32_t* wb = bits + sd.bmax / 32;
-  uint32_t* cur = wb + sd.bmax / 32;
-  for (uint32_t i = threadIdx.x; i < st.P; i += kThreads) lut[i] = lut_pool[sd.lut_off + i];
-  for (uint32_t i = threadIdx.x; i < nwords; i += kThreads) {
-    bits[i] = bits_pool[sd.bits_off + i];
-    wb[i] = wbase_pool[sd.bits_off + i];
-  }
+  char* base = reinterpret_cast<char*>(lds);
+  uint32_t* lut = reinterpret_cast<uint32_t*>(base);
+  uint2* bw = reinterpret_cast<uint2*>(base + (size_t)(1u << sd.pbits) * 4);  // (bits, wbase)
+  uint32_t* cur = reinterpret_cast<uint32_t*>(bw + (sd.bmax + 31) / 32);
+  KT* sval = reinterpret_cast<KT*>(cur + kCursorCap);
+  uint16_t* su = reinterpret_cast<uint16_t*>(sval + kWaves * kStageCap);
+  const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = lane_id();
+  for (uint32_t i = tid; i < st.P; i += kThreads) lut[i] = lut_pool[sd.lut_off + i];
+  for (uint32_t i = tid; i < nwords; i += kThreads)
+    bw[i] = make_uint2(bits_pool[sd.bits_off + i], wbase_pool[sd.bits_off + i]);
   uint32_t* row = rows + sd.row_base + (uint64_t)ch.idx * sd.bmax;
   const bool lds_cur = U <= kCursorCap;
   if (lds_cur) {
-    for (uint32_t u = threadIdx.x; u < U; u += kThreads) {
+    for (uint32_t u = tid; u < U; u += kThreads) {
       const Child& c = children[sd.q_off + u];
       cur[u] = (uint32_t)c.start + row[c.bucket];
     }
@@ -639,16 +695,94 @@ __global__ __launch_bounds__(kThreads) void k_compact(const KT* __restrict__ src
   Bucketer<KT> bk;
   bk.init(st);
   const Child* kids = children + sd.q_off;
-  for_each_elem<KT>(src + sd.data_off + ch.start, ch.len, [&](KT raw) {
-    uint32_t b = bk(to_key<C>(raw), lut);
-    uint32_t w = b >> 5, mw = bits[w];
-    if ((mw >> (b & 31)) & 1u) {
-      uint32_t u = wb[w] + __popc(mw & ((1u << (b & 31)) - 1u));
-      uint32_t pos = lds_cur ? atomicAdd(&cur[u], 1u)
-                             : (uint32_t)kids[u].start + atomicAdd(&row[b], 1u);
-      red[pos] = raw;
+  // child index of key `raw` or ~0u when its bucket is not active
+  auto child_of = [&](KT raw, uint32_t& b) -> uint32_t {
+    b = bk(to_key<C>(raw), lut);
+    const uint2 e = bw[b >> 5];
+    const uint32_t sh = b & 31u;
+    return ((e.x >> sh) & 1u) ? e.y + __popc(e.x & ((1u << sh) - 1u)) : ~0u;
+  };
+  auto place = [&](uint32_t u, uint32_t b, KT raw) {
+    uint32_t pos = lds_cur ? atomicAdd(&cur[u], 1u) : (uint32_t)kids[u].start + atomicAdd(&row[b], 1u);
+    red[pos] = raw;
+  };
+  const KT* p = src + sd.data_off + ch.start;
+  const uint64_t len = ch.len;
+  constexpr uint32_t VEC = 16 / sizeof(KT);
+  uint64_t mis = ((uintptr_t)p & 15u) / sizeof(KT);
+  uint64_t head = mis ? (VEC - mis) : 0;
+  if (head > len) head = len;
+  const uint64_t nvec = (len - head) / VEC;
+  const uint64_t done = head + nvec * VEC;
+  // head and tail: fewer than VEC keys each, placed directly (a thread may
+  // own one of each)
+  if (tid < head) {
+    const KT raw = p[tid];
+    uint32_t b, u = child_of(raw, b);
+    if (u != ~0u) place(u, b, raw);
+  }
+  if (done + tid < len) {
+    const KT raw = p[done + tid];
+    uint32_t b, u = child_of(raw, b);
+    if (u != ~0u) place(u, b, raw);
+  }
+  const uint4* __restrict__ pv = reinterpret_cast<const uint4*>(p + head);
+  KT* wv = sval + wave * kStageCap;
+  uint16_t* wu = su + wave * kStageCap;
+  uint32_t cnt = 0;  // staged keys of this wavefront (uniform)
+  auto flush = [&]() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (uint32_t i = lane; i < cnt; i += 64) {
+      uint32_t u = wu[i];
+      KT raw = wv[i];
+      uint32_t b = 0;
+      if (!lds_cur) b = kids[u].bucket;
+      place(u, b, raw);
     }
-  });
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    cnt = 0;
+  };
+  auto stage = [&](KT raw, bool valid) {
+    uint32_t b, u = child_of(raw, b);
+    const bool act = valid && u != ~0u;
+    const uint64_t m = __ballot(act);
+    if (act) {
+      const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                       __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+      wv[cnt + below] = raw;
+      wu[cnt + below] = (uint16_t)u;
+    }
+    cnt += (uint32_t)__popcll(m);
+  };
+  const uint64_t per_round = 4ull * kThreads;
+  const uint64_t rounds = (nvec + per_round - 1) / per_round;
+  for (uint64_t r = 0; r < rounds; ++r) {
+    uint4 v[4];
+    bool ok[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      uint64_t idx = r * per_round + (uint64_t)t * kThreads + tid;
+      ok[t] = idx < nvec;
+      v[t] = ok[t] ? pv[idx] : make_uint4(0, 0, 0, 0);
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      // a 16-byte group stages at most VEC * 64 keys per wavefront
+      if (cnt > kStageCap - VEC * 64) flush();
+      if (sizeof(KT) == 4) {
+        stage((KT)v[t].x, ok[t]); stage((KT)v[t].y, ok[t]);
+        stage((KT)v[t].z, ok[t]); stage((KT)v[t].w, ok[t]);
+      } else {
+        stage((KT)(((uint64_t)v[t].y << 32) | v[t].x), ok[t]);
+        stage((KT)(((uint64_t)v[t].w << 32) | v[t].z), ok[t]);
+      }
+    }
+  }
+  if (cnt) flush();
 }
 
 // Finisher: exact multi-rank radix select inside each child.

@@ -65,7 +65,9 @@ struct Plan {
   }
 };
 
-// Plan a single-vector select of nq ranks over n keys.
+// Plan a single-vector select of nq ranks over n keys.  The engine's single
+// segment reads its queries straight from the k list (or as ranks of a
+// top-K); the small-n direct path takes explicit Query records.
 inline void plan_single(Plan& p, int kind, int dtype, uint64_t n, uint32_t nq,
                         const ggks_opts& o) {
   const int eb = dtype_bytes(dtype);
@@ -111,6 +113,7 @@ inline void plan_single(Plan& p, int kind, int dtype, uint64_t n, uint32_t nq,
   const uint32_t bmax = 1u << bl;
   uint64_t G = o.blocks > 0 ? (uint64_t)o.blocks : t.chunks;
   G = std::max<uint64_t>(1, std::min<uint64_t>(G, (n + 4095) / 4096));
+  G = std::min<uint64_t>(G, kMaxChunks);
   uint64_t csz = (n + G - 1) / G;
   csz = (csz + 63) / 64 * 64;
   G = (n + csz - 1) / csz;
@@ -136,6 +139,8 @@ inline void plan_single(Plan& p, int kind, int dtype, uint64_t n, uint32_t nq,
   s.q_cnt = nq;
   s.mode = mode;
   s.radix_bits = rb;
+  s.qsrc = kind == GGKS_KIND_TOPK ? QSRC_TOPK : QSRC_K;
+  s.out_off = 0;
   p.segs.push_back(s);
   for (uint64_t i = 0; i < G; ++i) {
     Chunk c = {};
@@ -145,7 +150,7 @@ inline void plan_single(Plan& p, int kind, int dtype, uint64_t n, uint32_t nq,
     c.len = std::min<uint64_t>(csz, n - i * csz);
     p.chunks.push_back(c);
   }
-  for (uint32_t g = 0; g < bmax / kReduceBins; ++g) p.items.push_back({0u, g});
+  for (uint32_t g = 0; g < (bmax + kReduceBins - 1) / kReduceBins; ++g) p.items.push_back({0u, g});
   p.any_sample = mode == MODE_SAMPLE;
   p.max_nsample = ns;
   p.bmax_max = bmax;

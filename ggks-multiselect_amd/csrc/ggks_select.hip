@@ -53,6 +53,14 @@ struct ggks_ctx {
   size_t pinned_cap = 0;
   uint32_t* status = nullptr;  // device word
   std::mutex mu;
+  // stage timing (ggks_ctx_profile_arm)
+  std::vector<hipEvent_t> prof;  // nslots * (GGKS_NSTAGES + 1)
+  int prof_slots = 0;
+  uint64_t prof_calls = 0;
+  int prof_cur = -1;  // slot of the call in flight, -1 when not armed
+  // plan-table cache: the tables in the arena match this signature
+  uint64_t table_sig = 0;
+  bool pinned_busy = false;  // an async copy from `pinned` may be in flight
 };
 
 namespace ggks {
@@ -70,6 +78,7 @@ static int ensure_arena(ggks_ctx* c, size_t bytes) {
     return fail(GGKS_ENOMEM, "cannot allocate %zu bytes of device scratch", cap);
   }
   c->arena_cap = cap;
+  c->table_sig = 0;
   return GGKS_OK;
 }
 
@@ -86,6 +95,7 @@ static int ensure_pinned(ggks_ctx* c, size_t bytes) {
     return fail(GGKS_ENOMEM, "cannot allocate %zu bytes of pinned memory", cap);
   }
   c->pinned_cap = cap;
+  c->pinned_busy = false;
   return GGKS_OK;
 }
 
@@ -116,6 +126,14 @@ static int default_ctx(ggks_ctx** out) {
   return GGKS_OK;
 }
 
+// Record the event that opens `stage` (GGKS_NSTAGES closes the call).  Stages
+// a call skips are recorded at the same point, so they measure zero.
+static void mark(ggks_ctx* c, int stage) {
+  if (c->prof_cur < 0) return;
+  const int per = GGKS_NSTAGES + 1;
+  (void)hipEventRecord(c->prof[(size_t)c->prof_cur * per + stage], c->stream);
+}
+
 // ----------------------------------------------------------- kernel launch
 template <typename K>
 static void set_lds_limit(K kernel, size_t bytes) {
@@ -124,7 +142,8 @@ static void set_lds_limit(K kernel, size_t bytes) {
 }
 
 template <typename KT, int C>
-static int launch_level(ggks_ctx* c, const Plan& p, const KT* d_src, KT* d_out) {
+static int launch_level(ggks_ctx* c, const Plan& p, const KT* d_src, KT* d_out,
+                        const uint32_t* d_k) {
   char* A = c->arena;
   const SegDesc* segs = reinterpret_cast<const SegDesc*>(A + p.off_segs);
   SegState* states = reinterpret_cast<SegState*>(A + p.off_states);
@@ -145,30 +164,36 @@ static int launch_level(ggks_ctx* c, const Plan& p, const KT* d_src, KT* d_out) 
   hipStream_t s = c->stream;
   const uint32_t nseg = (uint32_t)p.segs.size();
 
+  mark(c, 1);
   if (p.any_sample) {
     dim3 g(nseg, (p.max_nsample + kThreads - 1) / kThreads);
     hipLaunchKernelGGL((k_sample<KT, C>), g, dim3(kThreads), 0, s, d_src, segs, samples, p.seed);
   }
+  mark(c, 2);
   hipLaunchKernelGGL((k_lut<KT>), dim3(nseg), dim3(kThreads), 0, s, segs, samples, states, lut);
+  mark(c, 3);
 
   const size_t lds_count = ((size_t)p.bmax_max + (1u << p.pbits_max)) * 4;
   set_lds_limit(k_count<KT, C>, lds_count);
   hipLaunchKernelGGL((k_count<KT, C>), dim3((uint32_t)p.chunks.size()), dim3(kThreads),
                      lds_count, s, d_src, segs, states, chunks, lut, rows);
-  hipLaunchKernelGGL(k_reduce, dim3((uint32_t)p.items.size()), dim3(256), 0, s, segs, states,
-                     items, rows, tot);
+  mark(c, 4);
+  hipLaunchKernelGGL(k_reduce, dim3((uint32_t)p.items.size()), dim3(kThreads), 0, s, segs,
+                     states, items, rows, tot);
+  mark(c, 5);
   const size_t lds_findk =
       ((size_t)p.bmax_max + (1u << p.pbits_max) + 2 * ((p.bmax_max + 31) / 32) + 32) * 4;
   set_lds_limit(k_findk<KT, C>, lds_findk);
   hipLaunchKernelGGL((k_findk<KT, C>), dim3(nseg), dim3(kThreads), lds_findk, s, segs, states,
-                     lut, tot, q, qb, qr, kids, cq, bits, wbase, d_out);
-  const size_t lds_compact =
-      ((size_t)(1u << p.pbits_max) + 2 * ((p.bmax_max + 31) / 32) + kCursorCap) * 4;
+                     lut, tot, q, d_k, c->status, qb, qr, kids, cq, bits, wbase, d_out);
+  const size_t lds_compact = compact_lds_bytes<KT>(p.pbits_max, p.bmax_max);
   set_lds_limit(k_compact<KT, C>, lds_compact);
+  mark(c, 6);
   hipLaunchKernelGGL((k_compact<KT, C>), dim3((uint32_t)p.chunks.size()), dim3(kThreads),
                      lds_compact, s, d_src, segs, states, chunks, lut, rows, bits, wbase, kids,
                      red);
   set_lds_limit(k_finish<KT, C>, kFinishLdsTotal);
+  mark(c, 7);
   const uint32_t fgrid = std::min<uint32_t>(p.nq, 2048);
   hipLaunchKernelGGL((k_finish<KT, C>), dim3(fgrid), dim3(kThreads), kFinishLdsTotal, s, d_src,
                      red, kids, p.nq, cq, d_out);
@@ -183,6 +208,7 @@ static int launch_direct(ggks_ctx* c, const Plan& p, const KT* d_src, KT* d_out)
   const Child* kids = reinterpret_cast<const Child*>(A + p.off_children);
   const Query* q = reinterpret_cast<const Query*>(A + p.off_q);
   set_lds_limit(k_finish<KT, C>, kFinishLdsTotal);
+  for (int st = 1; st <= 7; ++st) mark(c, st);
   hipLaunchKernelGGL((k_finish<KT, C>), dim3(1), dim3(kThreads), kFinishLdsTotal, c->stream,
                      d_src, (const KT*)nullptr, kids, 1u, q, d_out);
   HIP_TRY(hipGetLastError());
@@ -190,20 +216,41 @@ static int launch_direct(ggks_ctx* c, const Plan& p, const KT* d_src, KT* d_out)
 }
 
 template <typename KT, int C>
-static int launch_typed(ggks_ctx* c, const Plan& p, const void* d_src, void* d_out) {
+static int launch_typed(ggks_ctx* c, const Plan& p, const void* d_src, void* d_out,
+                        const uint32_t* d_k) {
   if (p.direct)
     return launch_direct<KT, C>(c, p, (const KT*)d_src, (KT*)d_out);
-  return launch_level<KT, C>(c, p, (const KT*)d_src, (KT*)d_out);
+  return launch_level<KT, C>(c, p, (const KT*)d_src, (KT*)d_out, d_k);
 }
 
-static int launch(ggks_ctx* c, int dtype, const Plan& p, const void* d_src, void* d_out) {
+static int launch(ggks_ctx* c, int dtype, const Plan& p, const void* d_src, void* d_out,
+                  const uint32_t* d_k) {
   switch (dtype) {
-    case GGKS_F32: return launch_typed<uint32_t, CONV_F>(c, p, d_src, d_out);
-    case GGKS_F64: return launch_typed<uint64_t, CONV_F>(c, p, d_src, d_out);
-    case GGKS_U32: return launch_typed<uint32_t, CONV_U>(c, p, d_src, d_out);
-    case GGKS_I32: return launch_typed<uint32_t, CONV_I>(c, p, d_src, d_out);
+    case GGKS_F32: return launch_typed<uint32_t, CONV_F>(c, p, d_src, d_out, d_k);
+    case GGKS_F64: return launch_typed<uint64_t, CONV_F>(c, p, d_src, d_out, d_k);
+    case GGKS_U32: return launch_typed<uint32_t, CONV_U>(c, p, d_src, d_out, d_k);
+    case GGKS_I32: return launch_typed<uint32_t, CONV_I>(c, p, d_src, d_out, d_k);
   }
   return fail(GGKS_EINVAL, "bad dtype %d", dtype);
+}
+
+// FNV-1a over the plan tables (decides whether the device copy is current).
+static uint64_t table_signature(const char* p, size_t n, const void* arena) {
+  uint64_t h = 1469598103934665603ull ^ (uint64_t)(uintptr_t)arena;
+  const uint64_t* w = reinterpret_cast<const uint64_t*>(p);
+  for (size_t i = 0; i < n / 8; ++i) h = (h ^ w[i]) * 1099511628211ull;
+  for (size_t i = n / 8 * 8; i < n; ++i) h = (h ^ (uint8_t)p[i]) * 1099511628211ull;
+  return h;
+}
+
+// The pinned staging buffer is about to be rewritten: wait for the copy that
+// may still be reading it.
+static int pinned_claim(ggks_ctx* c) {
+  if (c->pinned_busy) {
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->pinned_busy = false;
+  }
+  return GGKS_OK;
 }
 
 // -------------------------------------------------------------- one select
@@ -234,6 +281,7 @@ static int select_one(ggks_ctx* c, int kind, int dtype, const void* d_vec, int64
     if (!out) return fail(GGKS_EINVAL, "null output array");
   }
   if (num_k == 0) return GGKS_OK;
+  if (!c && (rc = default_ctx(&c))) return rc;
   ggks_opts o = {};
   if (opt) o = *opt;
   std::lock_guard<std::mutex> lk(c->mu);
@@ -253,32 +301,53 @@ static int select_one(ggks_ctx* c, int kind, int dtype, const void* d_vec, int64
   const size_t pin_total = pin_out + (host_out ? (size_t)num_k * eb : 0);
   if ((rc = ensure_pinned(c, pin_total))) return rc;
 
-  // plan tables -> pinned -> device (one copy)
-  p.write_tables(c->pinned);
-  HIP_TRY(hipMemcpyAsync(c->arena + p.off_tables, c->pinned, tbytes, hipMemcpyHostToDevice,
-                         c->stream));
-  Query* d_q = reinterpret_cast<Query*>(c->arena + p.off_q);
+  c->prof_cur = c->prof_slots > 0 ? (int)(c->prof_calls++ % (uint64_t)c->prof_slots) : -1;
+  mark(c, 0);
+  // plan tables: upload only when they differ from what the arena holds
+  {
+    std::vector<char> tab(tbytes);
+    p.write_tables(tab.data());
+    const uint64_t sig = table_signature(tab.data(), tbytes, c->arena);
+    if (sig != c->table_sig) {
+      if ((rc = pinned_claim(c))) return rc;
+      memcpy(c->pinned, tab.data(), tbytes);
+      HIP_TRY(hipMemcpyAsync(c->arena + p.off_tables, c->pinned, tbytes, hipMemcpyHostToDevice,
+                             c->stream));
+      c->pinned_busy = true;
+      c->table_sig = sig;
+    }
+  }
   const uint32_t nk = (uint32_t)num_k;
   const uint32_t qgrid = (nk + 255) / 256;
-  if (ksrc == KSource::TOPK) {
-    hipLaunchKernelGGL(k_topk_queries, dim3(qgrid), dim3(256), 0, c->stream, nk, (uint64_t)n, d_q);
-  } else {
-    const uint32_t* d_k = k;
-    if (ksrc == KSource::HOST) {
-      memcpy(c->pinned + pin_k, k, (size_t)num_k * 4);
-      HIP_TRY(hipMemcpyAsync(c->arena + off_dk, c->pinned + pin_k, (size_t)num_k * 4,
-                             hipMemcpyHostToDevice, c->stream));
-      d_k = reinterpret_cast<const uint32_t*>(c->arena + off_dk);
-    }
-    hipLaunchKernelGGL(k_queries, dim3(qgrid), dim3(256), 0, c->stream, d_k, nk, (uint64_t)n,
-                       d_q, c->status);
+  const uint32_t* d_k = k;
+  if (ksrc == KSource::HOST) {
+    if ((rc = pinned_claim(c))) return rc;
+    memcpy(c->pinned + pin_k, k, (size_t)num_k * 4);
+    HIP_TRY(hipMemcpyAsync(c->arena + off_dk, c->pinned + pin_k, (size_t)num_k * 4,
+                           hipMemcpyHostToDevice, c->stream));
+    c->pinned_busy = true;
+    d_k = reinterpret_cast<const uint32_t*>(c->arena + off_dk);
+  }
+  if (p.direct) {  // the direct path takes explicit Query records
+    Query* d_q = reinterpret_cast<Query*>(c->arena + p.off_q);
+    if (ksrc == KSource::TOPK)
+      hipLaunchKernelGGL(k_topk_queries, dim3(qgrid), dim3(256), 0, c->stream, nk, (uint64_t)n,
+                         d_q);
+    else
+      hipLaunchKernelGGL(k_queries, dim3(qgrid), dim3(256), 0, c->stream, d_k, nk, (uint64_t)n,
+                         d_q, c->status);
   }
   void* d_out = host_out ? (void*)(c->arena + off_dout) : out;
-  if ((rc = launch(c, dtype, p, d_vec, d_out))) return rc;
+  rc = launch(c, dtype, p, d_vec, d_out, d_k);
+  mark(c, GGKS_NSTAGES);
+  c->prof_cur = -1;
+  if (rc) return rc;
   if (host_out) {
+    // pin_out is disjoint from every host-written staging region
     HIP_TRY(hipMemcpyAsync(c->pinned + pin_out, d_out, (size_t)num_k * eb, hipMemcpyDeviceToHost,
                            c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
+    c->pinned_busy = false;
     memcpy(out, c->pinned + pin_out, (size_t)num_k * eb);
   }
   return GGKS_OK;
@@ -316,6 +385,41 @@ int ggks_ctx_destroy(ggks_ctx* c) {
 
 void* ggks_ctx_stream(ggks_ctx* c) { return c ? (void*)c->stream : nullptr; }
 
+int ggks_ctx_profile_arm(ggks_ctx* c, int nslots) {
+  if (!c || nslots < 0) return fail(GGKS_EINVAL, "bad profile arguments");
+  std::lock_guard<std::mutex> lk(c->mu);
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  for (hipEvent_t e : c->prof) (void)hipEventDestroy(e);
+  c->prof.clear();
+  c->prof_slots = 0;
+  c->prof_calls = 0;
+  const size_t n = (size_t)nslots * (GGKS_NSTAGES + 1);
+  for (size_t i = 0; i < n; ++i) {
+    hipEvent_t e;
+    HIP_TRY(hipEventCreate(&e));
+    c->prof.push_back(e);
+  }
+  c->prof_slots = nslots;
+  return GGKS_OK;
+}
+
+int ggks_ctx_profile_read(ggks_ctx* c, float* ms, int nslots) {
+  if (!c || !ms || nslots < 1 || nslots > c->prof_slots)
+    return fail(GGKS_EINVAL, "bad profile arguments");
+  std::lock_guard<std::mutex> lk(c->mu);
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  const int per = GGKS_NSTAGES + 1;
+  for (int sl = 0; sl < nslots; ++sl) {
+    hipEvent_t* e = &c->prof[(size_t)sl * per];
+    for (int st = 0; st < GGKS_NSTAGES; ++st)
+      HIP_TRY(hipEventElapsedTime(&ms[sl * per + st], e[st], e[st + 1]));
+    HIP_TRY(hipEventElapsedTime(&ms[sl * per + GGKS_NSTAGES], e[0], e[GGKS_NSTAGES]));
+  }
+  return GGKS_OK;
+}
+
 size_t ggks_scratch_bytes(int kind, int dtype, int64_t n, int64_t num_k) {
   if (dtype < 0 || dtype > 3 || n < 1 || num_k < 0) return 0;
   Plan p;
@@ -351,51 +455,37 @@ int ggks_ctx_check(ggks_ctx* c) {
   return GGKS_OK;
 }
 
-#define GGKS_CTX_OR_DEFAULT(c)                 \
-  do {                                         \
-    if (!(c)) {                                \
-      int rc_ = default_ctx(&(c));             \
-      if (rc_) return rc_;                     \
-    }                                          \
-  } while (0)
-
 int ggks_bucket_multiselect(ggks_ctx* c, int dtype, const void* d_vec, int64_t n,
                             const uint32_t* h_k, int64_t num_k, void* h_out,
                             const ggks_opts* opt) {
-  GGKS_CTX_OR_DEFAULT(c);
   return select_one(c, GGKS_KIND_BUCKET, dtype, d_vec, n, h_k, num_k, KSource::HOST, h_out, true,
                     opt);
 }
 int ggks_radix_multiselect(ggks_ctx* c, int dtype, const void* d_vec, int64_t n,
                            const uint32_t* h_k, int64_t num_k, void* h_out,
                            const ggks_opts* opt) {
-  GGKS_CTX_OR_DEFAULT(c);
   return select_one(c, GGKS_KIND_RADIX, dtype, d_vec, n, h_k, num_k, KSource::HOST, h_out, true,
                     opt);
 }
 int ggks_topk(ggks_ctx* c, int dtype, const void* d_vec, int64_t n, int64_t K, void* h_out,
               const ggks_opts* opt) {
-  GGKS_CTX_OR_DEFAULT(c);
   return select_one(c, GGKS_KIND_TOPK, dtype, d_vec, n, nullptr, K, KSource::TOPK, h_out, true,
                     opt);
 }
 int ggks_bucket_multiselect_async(ggks_ctx* c, int dtype, const void* d_vec, int64_t n,
                                   const uint32_t* d_k, int64_t num_k, void* d_out,
                                   const ggks_opts* opt) {
-  GGKS_CTX_OR_DEFAULT(c);
   return select_one(c, GGKS_KIND_BUCKET, dtype, d_vec, n, d_k, num_k, KSource::DEVICE, d_out,
                     false, opt);
 }
 int ggks_radix_multiselect_async(ggks_ctx* c, int dtype, const void* d_vec, int64_t n,
                                  const uint32_t* d_k, int64_t num_k, void* d_out,
                                  const ggks_opts* opt) {
-  GGKS_CTX_OR_DEFAULT(c);
   return select_one(c, GGKS_KIND_RADIX, dtype, d_vec, n, d_k, num_k, KSource::DEVICE, d_out,
                     false, opt);
 }
 int ggks_topk_async(ggks_ctx* c, int dtype, const void* d_vec, int64_t n, int64_t K, void* d_out,
                     const ggks_opts* opt) {
-  GGKS_CTX_OR_DEFAULT(c);
   return select_one(c, GGKS_KIND_TOPK, dtype, d_vec, n, nullptr, K, KSource::TOPK, d_out, false,
                     opt);
 }

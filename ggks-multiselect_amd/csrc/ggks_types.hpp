@@ -20,6 +20,14 @@ constexpr uint32_t kQCap = 1024;           // queries per finisher batch
 constexpr uint32_t kGroupMax = 32;         // prefix groups per histogram pass
 constexpr uint32_t kReduceBins = 64;       // bins per reduce work item
 constexpr uint32_t kSortRanks = 32;        // finisher sorts a child above this
+constexpr uint32_t kMaxChunks = 256;       // chunks per segment (reduce registers)
+
+// Where a segment's queries come from.
+enum QuerySource : uint32_t {
+  QSRC_QUERIES = 0,  // explicit Query records at q_off
+  QSRC_K = 1,        // raw k values (k-th largest) at q_off; output slot out_off + i
+  QSRC_TOPK = 2      // the q_cnt largest: rank len - i into output slot out_off + i
+};
 
 // Bucket-function modes of a segment.
 enum SegMode : uint32_t {
@@ -52,6 +60,8 @@ struct SegDesc {          // planned on the host, read-only on the device
   uint32_t q_off, q_cnt;  // its queries; child slots use the same range
   uint32_t mode;          // SegMode
   uint32_t radix_bits;    // MODE_RADIX digit width
+  uint32_t qsrc;          // QuerySource
+  uint32_t out_off;       // first output slot (QSRC_K, QSRC_TOPK)
 };
 
 struct SegState {         // written on the device by the LUT kernel

@@ -60,7 +60,7 @@ typedef struct ggks_opts {
   int32_t buckets_log2; /* bucket budget as log2 (8..15); 0 = tuned */
   int32_t destructive;  /* accepted; the input is never modified */
   uint64_t seed;        /* seed of the splitter sample; 0 = fixed default */
-  int32_t radix_bits;   /* radix path: digit width per level (4..15); 0 = 15 */
+  int32_t radix_bits;   /* radix path: digit width per level (4..14); 0 = 14 */
   int32_t reserved[5];
 } ggks_opts;
 
@@ -131,6 +131,18 @@ int ggks_ctx_reserve(ggks_ctx *ctx, int kind, int dtype, int64_t n,
  * a k outside [1, n] since the last check (the flag is then cleared). */
 int ggks_ctx_check(ggks_ctx *ctx);
 void *ggks_ctx_stream(ggks_ctx *ctx);
+
+/* ---- stage timing ---------------------------------------------------
+ * When armed with nslots > 0, every select on the context records HIP events
+ * on its stream around each kernel stage into slot (call index % nslots).
+ * After the stream drained, ggks_ctx_profile_read() fills
+ * ms[slot * (GGKS_NSTAGES + 1) + stage] (stage durations) and
+ * ms[slot * (GGKS_NSTAGES + 1) + GGKS_NSTAGES] (whole call, first event to
+ * last).  Stages: 0 queries, 1 sample, 2 lut, 3 count (pass 1), 4 reduce,
+ * 5 findk, 6 compact (pass 2), 7 finish.  nslots = 0 disarms. */
+#define GGKS_NSTAGES 8
+int ggks_ctx_profile_arm(ggks_ctx *ctx, int nslots);
+int ggks_ctx_profile_read(ggks_ctx *ctx, float *ms, int nslots);
 
 /* ---- device plumbing for hosts without a GPU framework --------------
  * Thin wrappers over the HIP runtime this library is linked with. */
