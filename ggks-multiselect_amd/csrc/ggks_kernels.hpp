@@ -36,14 +36,23 @@ namespace ggks {
 
 __device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
 
+// Inclusive prefix sum across a wavefront with DPP moves (no LDS traffic):
+// row_shr 1, 2, 4, 8 scan each row of 16 lanes, then row_bcast 15 and 31
+// carry the row totals into rows 1, 3 and 2, 3.  Lanes without a source
+// receive 0.
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
-  const uint32_t lane = lane_id();
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    uint32_t t = __shfl_up(v, d, 64);
-    if (lane >= (uint32_t)d) v += t;
-  }
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false);  // row_shr:1
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false);  // row_shr:2
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false);  // row_shr:4
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false);  // row_shr:8
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);  // row_bcast:15
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);  // row_bcast:31
   return v;
+}
+
+// Value of lane 63 (the wavefront total after an inclusive scan), uniform.
+__device__ __forceinline__ uint32_t wave_last(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
 }
 
 // Block-wide exclusive scan of one value per thread (blockDim.x == 1024).
@@ -67,20 +76,35 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* wsum,
   return res;
 }
 
-// In-place exclusive scan of a[0, L) in LDS by one 1024-thread block.
+// In-place exclusive scan of a[0, L) in LDS by one 1024-thread block.  Each
+// wavefront owns a contiguous segment and walks it 64 consecutive words at a
+// time (no bank conflicts); wavefront totals are scanned once.  wsum needs 17
+// words.  Returns the total.
 __device__ uint32_t block_scan_array(uint32_t* a, uint32_t L, uint32_t* wsum) {
-  const uint32_t per = (L + kThreads - 1) / kThreads;
-  const uint32_t b0 = threadIdx.x * per;
-  const uint32_t b1 = min(L, b0 + per);
-  uint32_t local = 0;
-  for (uint32_t i = b0; i < b1; ++i) local += a[i];
-  uint32_t total;
-  uint32_t run = block_excl_scan(local, wsum, total);
-  for (uint32_t i = b0; i < b1; ++i) {
-    uint32_t v = a[i];
-    a[i] = run;
-    run += v;
+  const uint32_t w = threadIdx.x >> 6, lane = lane_id();
+  constexpr uint32_t NW = kThreads / 64;
+  const uint32_t seg = ((L + NW - 1) / NW + 63) / 64 * 64;
+  const uint32_t b0 = w * seg, b1 = min(L, b0 + seg);
+  uint32_t carry = 0;
+  for (uint32_t base = b0; base < b1; base += 64) {
+    const uint32_t i = base + lane;
+    const uint32_t v = i < b1 ? a[i] : 0u;
+    const uint32_t inc = wave_incl_scan(v);
+    if (i < b1) a[i] = carry + inc - v;
+    carry += wave_last(inc);
   }
+  if (lane == 0) wsum[w] = carry;
+  __syncthreads();
+  if (w == 0) {
+    uint32_t t = lane < NW ? wsum[lane] : 0u;
+    uint32_t ti = wave_incl_scan(t);
+    if (lane < NW) wsum[lane] = ti - t;
+    if (lane == NW - 1) wsum[NW] = ti;
+  }
+  __syncthreads();
+  const uint32_t add = wsum[w];
+  for (uint32_t i = b0 + lane; i < b1; i += 64) a[i] += add;
+  const uint32_t total = wsum[NW];
   __syncthreads();
   return total;
 }
@@ -117,95 +141,95 @@ __device__ __forceinline__ int bitlen(KT v) {
   return v == 0 ? 0 : (sizeof(KT) == 8 ? 64 - __clzll((long long)v) : 32 - __clz((int)v));
 }
 
-// The bucket function of a segment.  bucket 0 holds keys below kb, bucket B-1
-// keys above the last prefix; interior bucket = 1 + base[p] + j with
-// p = (key - kb) >> s and j = floor(sub * nb[p] / 2^s), sub the low s bits.
-// j is computed as mulhi(sub << (W - s), nb) which is exact.
+// The bucket function of a segment (field layout in SegState).  Every step
+// is a select, min/max, bit-field extract or a full-rate 24-bit multiply, so
+// a key costs about a dozen VALU instructions and one LDS read.  It is
+// monotone in the key: the prefix is, and inside a prefix the sub-bucket
+// floor(t * nb / 2^24) is monotone in t.  Keys below the sampled minimum or
+// above the last prefix land in the first or last bucket.
 template <typename KT>
-struct Bucketer {
-  KT kb, smask;
-  uint32_t s1, s2, P, Bm1, lsh;
+struct BucketFn {
+  KT kb, lmax;
+  uint32_t poff, pwid, rsh, twid, lsh;
   __device__ __forceinline__ void init(const SegState& st) {
-    constexpr uint32_t W = KeyBits<KT>::W;
     kb = (KT)st.kb;
-    s1 = st.s >> 1;
-    s2 = st.s - s1;
-    P = st.P;
-    Bm1 = st.B - 1;
-    smask = st.s >= W ? KeyBits<KT>::MAX : ((KT(1) << st.s) - KT(1));
-    lsh = st.s == 0 ? 0u : (W - st.s);
+    lmax = (KT)st.lmax;
+    poff = st.poff;
+    pwid = st.pwid;
+    rsh = st.rsh;
+    twid = st.twid;
+    lsh = st.lsh;
   }
-  // Branch-free: the two tail cases are selected after the look-up, so a
-  // wavefront issues all its LDS reads without exec-mask changes.
   __device__ __forceinline__ uint32_t operator()(KT key, const uint32_t* lut) const {
-    const bool below = key < kb;
-    const KT low = key - kb;
-    const KT p = (low >> s1) >> s2;
-    const bool above = p >= (KT)P;
-    const uint32_t pi = (below | above) ? 0u : (uint32_t)p;
-    const uint32_t e = lut[pi];
-    const uint32_t j = mulhi_nb<KT>((low & smask) << lsh, e >> 16);
-    const uint32_t b = 1u + (e & 0xFFFFu) + j;
-    return below ? 0u : (above ? Bm1 : b);
+    KT low = (key > kb ? key : kb) - kb;
+    low = low < lmax ? low : lmax;
+    uint32_t p, t;
+    if (sizeof(KT) == 4) {
+      p = __builtin_amdgcn_ubfe((uint32_t)low, poff, pwid);
+      t = __builtin_amdgcn_ubfe((uint32_t)low, rsh, twid) << lsh;
+    } else {
+      const uint64_t l64 = (uint64_t)low;
+      p = (uint32_t)(l64 >> poff) & ((1u << pwid) - 1u);
+      t = ((uint32_t)(l64 >> rsh) & ((1u << twid) - 1u)) << lsh;
+    }
+    const uint32_t e = lut[p];
+    const uint32_t nbsh = (e >> 16) << 8;  // nb * 2^8 < 2^24
+    const uint32_t j = (uint32_t)(((uint64_t)(t & 0xFFFFFFu) * (nbsh & 0xFFFFFFu)) >> 32);
+    return (e & 0xFFFFu) + j;
   }
 };
 
-// ceil(j * 2^s / nb) for j < nb < 2^16, s <= 64, without overflow.
-__device__ __forceinline__ uint64_t ceil_j_pow2_div(uint32_t j, uint32_t s, uint32_t nb) {
-  if (j == 0 || nb <= 1) return 0;
-  if (s <= 47) {
-    uint64_t num = ((uint64_t)j << s) + nb - 1;
-    return num / nb;
-  }
-  // j*2^s = (j*2^16) * 2^(s-16); split j*2^16 = q*nb + r.
-  uint64_t A = (uint64_t)j << 16;
-  uint64_t q = A / nb, r = A % nb;
-  uint32_t t = s - 16;  // <= 48
-  return (q << t) + (((r << t) + nb - 1) / nb);
-}
-
-// Key range [lo, hi] of bucket b of a segment (b must hold elements).
-// Interior buckets: lo is the smallest key mapped to b.  Because the bucket
-// function is monotone, hi is the smallest key of the next bucket minus one.
+// Key range [lo, hi] of a bucket, found by binary search over keys with the
+// same device function the passes use, so it can never disagree with them.
 template <typename KT>
 struct RangeOf {
-  const SegState* st;
+  BucketFn<KT> fn;
   const uint32_t* lut;
-  __device__ unsigned __int128 first_key(uint32_t bi) const {
-    // smallest prefix p with base[p] + max(nb[p], 1) > bi
-    uint32_t lo = 0, hi = st->P - 1;
+  uint32_t B, P, s;
+  // Closed-form candidate for the smallest key of bucket b: find the prefix p
+  // holding b, then the smallest t with floor(t * nb / 2^24) >= j and the
+  // smallest sub-field value giving that t.
+  __device__ bool candidate(uint32_t b, KT& out) const {
+    uint32_t lo = 0, hi = P - 1;  // smallest prefix whose bucket range ends past b
     while (lo < hi) {
       uint32_t mid = (lo + hi) >> 1;
       uint32_t e = lut[mid];
       uint32_t nb = e >> 16;
-      uint32_t end = (e & 0xFFFFu) + (nb > 1 ? nb : 1u);
-      if (end > bi) hi = mid; else lo = mid + 1;
+      if ((e & 0xFFFFu) + (nb > 1 ? nb : 1u) > b) hi = mid; else lo = mid + 1;
     }
-    uint32_t e = lut[lo];
-    uint32_t nb = e >> 16;
-    uint32_t j = bi - (e & 0xFFFFu);
-    return (unsigned __int128)st->kb + ((unsigned __int128)lo << st->s) +
-           ceil_j_pow2_div(j, st->s, nb);
+    const uint32_t e = lut[lo];
+    const uint32_t nb = e >> 16, j = b - (e & 0xFFFFu);
+    uint64_t sub = 0;
+    if (j > 0 && nb > 1) {
+      const uint64_t T = (((uint64_t)j << 24) + nb - 1) / nb;  // smallest t
+      sub = fn.rsh ? (T << fn.rsh) : ((T + (1ull << fn.lsh) - 1) >> fn.lsh);
+    }
+    const unsigned __int128 low = ((unsigned __int128)lo << s) + sub;
+    if (low > (unsigned __int128)fn.lmax) return false;
+    out = (KT)((KT)low + fn.kb);
+    return true;
+  }
+  // smallest key whose bucket is >= b, or false if no key reaches b
+  __device__ bool first_key(uint32_t b, KT& out) const {
+    if (b == 0) { out = 0; return true; }
+    KT c;
+    if (candidate(b, c) && fn(c, lut) >= b && (c == 0 || fn((KT)(c - 1), lut) < b)) {
+      out = c;
+      return true;
+    }
+    if (fn(KeyBits<KT>::MAX, lut) < b) return false;
+    KT lo = 0, hi = KeyBits<KT>::MAX;
+    while (lo < hi) {
+      KT mid = lo + ((hi - lo) >> 1);
+      if (fn(mid, lut) >= b) hi = mid; else lo = mid + 1;
+    }
+    out = lo;
+    return true;
   }
   __device__ void operator()(uint32_t b, KT& lo, KT& hi) const {
-    constexpr unsigned __int128 KMAX = (unsigned __int128)KeyBits<KT>::MAX;
-    unsigned __int128 end = (unsigned __int128)st->kb + ((unsigned __int128)st->P << st->s);
-    if (b == 0) {
-      lo = 0;
-      hi = (KT)(st->kb - 1);
-      return;
-    }
-    if (b == st->B - 1) {
-      lo = (KT)(end > KMAX ? KMAX : end);
-      hi = KeyBits<KT>::MAX;
-      return;
-    }
-    uint32_t bi = b - 1;
-    unsigned __int128 l = first_key(bi);
-    unsigned __int128 h = (bi + 1 < st->Bi) ? first_key(bi + 1) - 1 : end - 1;
-    if (h > KMAX) h = KMAX;
-    lo = (KT)l;
-    hi = (KT)h;
+    first_key(b, lo);
+    KT nxt;
+    hi = (b + 1 < B && first_key(b + 1, nxt)) ? (KT)(nxt - 1) : KeyBits<KT>::MAX;
   }
 };
 
@@ -296,7 +320,6 @@ __global__ __launch_bounds__(kThreads) void k_lut(const SegDesc* __restrict__ se
   const SegDesc sd = segs[blockIdx.x];
   const uint32_t tid = threadIdx.x;
   uint32_t* lut = lut_pool + sd.lut_off;
-  SegState st = {};
   KT kmin, kmax;
   uint32_t pb = sd.pbits;
 
@@ -329,8 +352,7 @@ __global__ __launch_bounds__(kThreads) void k_lut(const SegDesc* __restrict__ se
   const int bl = bitlen<KT>(R);
   uint32_t s, P;
   if (sd.mode == MODE_RADIX) {
-    uint32_t rb = sd.radix_bits;
-    pb = rb < pb ? rb : pb;
+    pb = sd.radix_bits < pb ? sd.radix_bits : pb;
     s = W - pb;
     P = 1u << pb;
   } else if (sd.mode == MODE_RANGE) {
@@ -339,20 +361,21 @@ __global__ __launch_bounds__(kThreads) void k_lut(const SegDesc* __restrict__ se
     P = 1;
   } else {
     s = bl > (int)pb ? (uint32_t)(bl - (int)pb) : 0u;
-    P = (uint32_t)(((uint64_t)R >> s) + 1);  // s >= 1 whenever R >= 2^pb
+    P = (uint32_t)(((uint64_t)R >> s) + 1);  // <= 2^pb
   }
-  const uint32_t Bt = sd.bmax - 2;  // interior budget
+  const uint32_t Bt = sd.bmax;  // bucket budget
   const uint32_t cap = s >= 16 ? 65535u : min(65535u, 1u << s);
+  uint32_t Btot;
 
   if (sd.mode == MODE_RADIX) {
-    // every prefix gets 2^(radix_bits - pb) buckets: the next bits exactly
-    uint32_t nb = 1u << (sd.radix_bits - pb);
+    // every prefix gets 2^(radix_bits - pb) buckets: exactly the next bits
+    const uint32_t nb = 1u << (sd.radix_bits - pb);
     for (uint32_t p = tid; p < P; p += kThreads) lut[p] = (p * nb) | (nb << 16);
-    st.Bi = P * nb;
+    Btot = P * nb;
   } else if (sd.mode == MODE_RANGE) {
-    uint32_t nb = min(Bt, cap);
+    const uint32_t nb = min(Bt, cap);
     if (tid == 0) lut[0] = nb << 16;
-    st.Bi = nb;
+    Btot = nb;
   } else {
     for (uint32_t p = tid; p < P; p += kThreads) cnt[p] = 0;
     __syncthreads();
@@ -371,37 +394,43 @@ __global__ __launch_bounds__(kThreads) void k_lut(const SegDesc* __restrict__ se
     block_excl_scan(runs, wsum, t2);
     const uint32_t A = Bt > t1 + t2 ? Bt - t1 - t2 : 0u;
     const uint32_t ns = sd.nsample;
-    // width of each prefix in buckets, then an exclusive scan gives the base
     const uint32_t per = (P + kThreads - 1) / kThreads;
     const uint32_t p0 = tid * per, p1 = min(P, p0 + per);
-    uint32_t local = 0;
-    for (uint32_t p = p0; p < p1; ++p) {
-      uint32_t c = cnt[p];
-      uint32_t w;
-      if (c) w = min(cap, 1u + (uint32_t)(((uint64_t)c * A) / ns));
-      else w = (p == 0 || cnt[p - 1]) ? 1u : 0u;
-      local += w;
-    }
+    auto width = [&](uint32_t p, uint32_t& nb) -> uint32_t {
+      const uint32_t c = cnt[p];
+      if (c) { nb = min(cap, 1u + (uint32_t)(((uint64_t)c * A) / ns)); return nb; }
+      nb = 0;
+      return (p == 0 || cnt[p - 1]) ? 1u : 0u;
+    };
+    uint32_t local = 0, nbv;
+    for (uint32_t p = p0; p < p1; ++p) local += width(p, nbv);
     uint32_t tot;
     uint32_t run = block_excl_scan(local, wsum, tot);
     for (uint32_t p = p0; p < p1; ++p) {
-      uint32_t c = cnt[p];
-      uint32_t nb, w, base;
-      if (c) { nb = min(cap, 1u + (uint32_t)(((uint64_t)c * A) / ns)); w = nb; base = run; }
-      else if (p == 0 || cnt[p - 1]) { nb = 0; w = 1; base = run; }
-      else { nb = 0; w = 0; base = run - 1; }
-      lut[p] = base | (nb << 16);
+      const uint32_t w = width(p, nbv);
+      const uint32_t base = (w == 0) ? run - 1 : run;  // a run's later prefixes
+      lut[p] = base | (nbv << 16);
       run += w;
     }
-    st.Bi = tot;
+    Btot = tot;
   }
   if (tid == 0) {
+    SegState st = {};
     st.kb = (uint64_t)kmin;
     st.kmax = (uint64_t)kmax;
+    // lmax = min(P * 2^s - 1, KMAX - kb): every key maps into a prefix < P
+    const unsigned __int128 span = ((unsigned __int128)P << s) - 1;
+    const unsigned __int128 room = (unsigned __int128)(KeyBits<KT>::MAX - kmin);
+    st.lmax = (uint64_t)(span < room ? span : room);
     st.s = s;
     st.P = P;
-    st.B = st.Bi + 2;
+    st.B = Btot;
     st.nact = 0;
+    st.pwid = P > 1 ? pb : 0;
+    st.poff = P > 1 ? s : 0;
+    st.twid = s < 24 ? s : 24;
+    st.rsh = s > 24 ? s - 24 : 0;
+    st.lsh = s < 24 ? 24 - s : 0;
     states[blockIdx.x] = st;
   }
 }
@@ -424,7 +453,7 @@ __global__ __launch_bounds__(kThreads) void k_count(const KT* __restrict__ src,
   for (uint32_t i = threadIdx.x; i < st.B; i += kThreads) hist[i] = 0;
   for (uint32_t i = threadIdx.x; i < st.P; i += kThreads) lut[i] = lut_pool[sd.lut_off + i];
   __syncthreads();
-  Bucketer<KT> bk;
+  BucketFn<KT> bk;
   bk.init(st);
   for_each_elem<KT>(src + sd.data_off + ch.start, ch.len, [&](KT raw) {
     atomicAdd(&hist[bk(to_key<C>(raw), lut)], 1u);
@@ -483,7 +512,6 @@ __global__ __launch_bounds__(kThreads) void k_reduce(const SegDesc* __restrict__
   if (valid && q == 0) totals[sd.tot_off + bin] = total;
 }
 
-// Per segment: ranks -> buckets, children, and the ranks grouped by child.
 // The i-th query of a segment: explicit, from a raw k list (k-th largest ->
 // rank len - k + 1, output slot i; a bad k raises a status bit and is clamped
 // to k = 1), or rank len - i for a top-K.
@@ -506,6 +534,8 @@ __device__ __forceinline__ Query seg_query(const SegDesc& sd, const Query* __res
   return q;
 }
 
+// Per segment: ranks -> buckets, children, the child map for the compaction
+// pass, and the ranks grouped by child.
 template <typename KT, int C>
 __global__ __launch_bounds__(kThreads) void k_findk(const SegDesc* __restrict__ segs,
                                                     SegState* __restrict__ states,
@@ -518,8 +548,7 @@ __global__ __launch_bounds__(kThreads) void k_findk(const SegDesc* __restrict__ 
                                                     uint32_t* __restrict__ qr,
                                                     Child* __restrict__ children,
                                                     Query* __restrict__ cq,
-                                                    uint32_t* __restrict__ bits_pool,
-                                                    uint32_t* __restrict__ wbase_pool,
+                                                    uint16_t* __restrict__ amap_pool,
                                                     KT* __restrict__ out) {
   extern __shared__ uint32_t lds[];
   const SegDesc sd = segs[blockIdx.x];
@@ -528,18 +557,25 @@ __global__ __launch_bounds__(kThreads) void k_findk(const SegDesc* __restrict__ 
   __syncthreads();
   const uint32_t B = st.B;
   const uint32_t nwords = (B + 31) / 32;
+  const uint32_t wcap = (sd.bmax + 31) / 32;
   uint32_t* hist = lds;                          // bmax
   uint32_t* lut = hist + sd.bmax;                // 2^pbits
-  uint32_t* bits = lut + (1u << sd.pbits);       // bmax / 32
-  uint32_t* wb = bits + sd.bmax / 32;            // bmax / 32
-  uint32_t* wsum = wb + sd.bmax / 32;            // 17 (+ pad)
+  uint32_t* bits = lut + (1u << sd.pbits);       // wcap
+  uint32_t* wb = bits + wcap;                    // wcap: first child of a word
+  uint32_t* wc = wb + wcap;                      // wcap: first reduced slot of a word
+  uint32_t* wsum = wc + wcap;                    // 17 (+ pad)
   const uint32_t tid = threadIdx.x;
   strided_for(totals + sd.tot_off, B, [&](uint32_t i, uint32_t v) { hist[i] = v; });
   strided_for(lut_pool + sd.lut_off, st.P, [&](uint32_t i, uint32_t v) { lut[i] = v; });
   for (uint32_t i = tid; i < nwords; i += kThreads) bits[i] = 0;
   __syncthreads();
   const uint32_t len = block_scan_array(hist, B, wsum);  // hist = exclusive
-  RangeOf<KT> range{&st, lut};
+  RangeOf<KT> range;
+  range.fn.init(st);
+  range.lut = lut;
+  range.B = B;
+  range.P = st.P;
+  range.s = st.s;
   auto count_of = [&](uint32_t b) { return (b + 1 < B ? hist[b + 1] : len) - hist[b]; };
 
   // A: bucket of every rank; single-key buckets are answered here
@@ -565,7 +601,7 @@ __global__ __launch_bounds__(kThreads) void k_findk(const SegDesc* __restrict__ 
   }
   __syncthreads();
 
-  // B: active-bucket index per bitmap word, child layout in bucket order
+  // B: per bitmap word its first child and first reduced slot; the child map
   uint32_t pc = 0, csum = 0, m = 0;
   if (tid < nwords) {
     m = bits[tid];
@@ -577,29 +613,41 @@ __global__ __launch_bounds__(kThreads) void k_findk(const SegDesc* __restrict__ 
   const uint32_t cbase = block_excl_scan(csum, wsum, mtot);
   if (tid < nwords) {
     wb[tid] = ubase;
-    bits_pool[sd.bits_off + tid] = m;
-    wbase_pool[sd.bits_off + tid] = ubase;
-    uint32_t u = ubase;
-    uint64_t start = sd.red_base + cbase;
-    for (uint32_t mm = m; mm; mm &= mm - 1) {
-      uint32_t b = tid * 32 + __ffs(mm) - 1;
-      Child c;
-      c.start = start;
-      c.len = count_of(b);
-      KT klo, khi;
-      range(b, klo, khi);
-      c.lo = klo;
-      c.hi = khi;
-      c.q_off = 0;
-      c.q_cnt = 0;
-      c.src = 1;
-      c.need_range = 0;
-      c.bucket = b;
-      c.seg = blockIdx.x;
-      children[sd.q_off + u] = c;
-      start += c.len;
-      ++u;
+    wc[tid] = cbase;
+  }
+  __syncthreads();
+  for (uint32_t b = tid; b < B; b += kThreads) {  // consecutive threads, consecutive slots
+    const uint32_t mw = bits[b >> 5], sh = b & 31u;
+    amap_pool[sd.amap_off + b] =
+        ((mw >> sh) & 1u) ? (uint16_t)(wb[b >> 5] + __popc(mw & ((1u << sh) - 1u))) : kNoChild;
+  }
+  // one thread per child: its bucket, reduced slice and key range
+  for (uint32_t u = tid; u < U; u += kThreads) {
+    uint32_t a = 0, z = nwords - 1;  // largest word with wb <= u
+    while (a < z) {
+      uint32_t mid = (a + z + 1) >> 1;
+      if (wb[mid] <= u) a = mid; else z = mid - 1;
     }
+    uint32_t mw = bits[a], start = wc[a];
+    for (uint32_t r = u - wb[a]; r; --r) {  // skip earlier children of the word
+      start += count_of(a * 32 + __ffs(mw) - 1);
+      mw &= mw - 1;
+    }
+    const uint32_t b = a * 32 + __ffs(mw) - 1;
+    Child c;
+    c.start = sd.red_base + start;
+    c.len = count_of(b);
+    KT klo, khi;
+    range(b, klo, khi);
+    c.lo = klo;
+    c.hi = khi;
+    c.q_off = 0;
+    c.q_cnt = 0;
+    c.src = 1;
+    c.need_range = 0;
+    c.bucket = b;
+    c.seg = blockIdx.x;
+    children[sd.q_off + u] = c;
   }
   __syncthreads();
 
@@ -642,17 +690,20 @@ __global__ __launch_bounds__(kThreads) void k_findk(const SegDesc* __restrict__ 
 
 // Pass 2: copy the keys of active buckets into their children's ranges.
 //
-// Every wavefront stages its active keys in LDS (slot = ballot prefix count,
-// no atomics) and flushes them in batches: one LDS cursor atomic per staged
-// key, 64 lanes at a time, then the global store.  The body runs in uniform
-// rounds so every lane of a wavefront takes part in every ballot.
-constexpr uint32_t kStageCap = 512;  // staged keys per wavefront (>= 2 groups of 256)
+// A thread maps its keys to children through the LDS child map.  Active keys
+// are staged per wavefront in LDS (slot = wavefront prefix sum of per-lane
+// counts, no atomics) and flushed in batches: one LDS cursor atomic per key,
+// 64 lanes at a time, then the global store.  Rounds are uniform so every lane
+// of a wavefront takes part in every shuffle.
 constexpr uint32_t kWaves = kThreads / 64;
+// staged keys per wavefront: two 16-byte groups of the whole wavefront
+template <typename KT>
+constexpr uint32_t stage_cap() { return 2 * 64 * (16 / sizeof(KT)); }
 
 template <typename KT>
 __host__ __device__ constexpr size_t compact_lds_bytes(uint32_t pbits, uint32_t bmax) {
-  return (size_t)(1u << pbits) * 4 + (size_t)((bmax + 31) / 32) * 8 + (size_t)kCursorCap * 4 +
-         (size_t)kWaves * kStageCap * (sizeof(KT) + 2);
+  return (size_t)(1u << pbits) * 4 + (size_t)bmax * 2 + (size_t)kCursorCap * 4 +
+         (size_t)kWaves * stage_cap<KT>() * (sizeof(KT) + 2);
 }
 
 template <typename KT, int C>
@@ -662,8 +713,7 @@ __global__ __launch_bounds__(kThreads) void k_compact(const KT* __restrict__ src
                                                       const Chunk* __restrict__ chunks,
                                                       const uint32_t* __restrict__ lut_pool,
                                                       uint32_t* __restrict__ rows,
-                                                      const uint32_t* __restrict__ bits_pool,
-                                                      const uint32_t* __restrict__ wbase_pool,
+                                                      const uint16_t* __restrict__ amap_pool,
                                                       const Child* __restrict__ children,
                                                       KT* __restrict__ red) {
   extern __shared__ __align__(16) uint32_t lds[];
@@ -672,38 +722,32 @@ __global__ __launch_bounds__(kThreads) void k_compact(const KT* __restrict__ src
   const SegState st = states[ch.seg];
   const uint32_t U = st.nact;
   if (U == 0) return;
-  const uint32_t nwords = (st.B + 31) / 32;
   char* base = reinterpret_cast<char*>(lds);
   uint32_t* lut = reinterpret_cast<uint32_t*>(base);
-  uint2* bw = reinterpret_cast<uint2*>(base + (size_t)(1u << sd.pbits) * 4);  // (bits, wbase)
-  uint32_t* cur = reinterpret_cast<uint32_t*>(bw + (sd.bmax + 31) / 32);
+  uint16_t* amap = reinterpret_cast<uint16_t*>(base + (size_t)(1u << sd.pbits) * 4);
+  uint32_t* cur = reinterpret_cast<uint32_t*>(amap + (sd.bmax + 1) / 2 * 2);
   KT* sval = reinterpret_cast<KT*>(cur + kCursorCap);
+  constexpr uint32_t kStageCap = stage_cap<KT>();
   uint16_t* su = reinterpret_cast<uint16_t*>(sval + kWaves * kStageCap);
   const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = lane_id();
   for (uint32_t i = tid; i < st.P; i += kThreads) lut[i] = lut_pool[sd.lut_off + i];
-  for (uint32_t i = tid; i < nwords; i += kThreads)
-    bw[i] = make_uint2(bits_pool[sd.bits_off + i], wbase_pool[sd.bits_off + i]);
-  uint32_t* row = rows + sd.row_base + (uint64_t)ch.idx * sd.bmax;
-  const bool lds_cur = U <= kCursorCap;
-  if (lds_cur) {
-    for (uint32_t u = tid; u < U; u += kThreads) {
-      const Child& c = children[sd.q_off + u];
-      cur[u] = (uint32_t)c.start + row[c.bucket];
-    }
+  {
+    const uint32_t* a32 = reinterpret_cast<const uint32_t*>(amap_pool + sd.amap_off);
+    uint32_t* d32 = reinterpret_cast<uint32_t*>(amap);
+    strided_for(a32, (st.B + 1) / 2, [&](uint32_t i, uint32_t v) { d32[i] = v; });
   }
-  __syncthreads();
-  Bucketer<KT> bk;
-  bk.init(st);
+  uint32_t* row = rows + sd.row_base + (uint64_t)ch.idx * sd.bmax;
   const Child* kids = children + sd.q_off;
-  // child index of key `raw` or ~0u when its bucket is not active
-  auto child_of = [&](KT raw, uint32_t& b) -> uint32_t {
-    b = bk(to_key<C>(raw), lut);
-    const uint2 e = bw[b >> 5];
-    const uint32_t sh = b & 31u;
-    return ((e.x >> sh) & 1u) ? e.y + __popc(e.x & ((1u << sh) - 1u)) : ~0u;
-  };
-  auto place = [&](uint32_t u, uint32_t b, KT raw) {
-    uint32_t pos = lds_cur ? atomicAdd(&cur[u], 1u) : (uint32_t)kids[u].start + atomicAdd(&row[b], 1u);
+  for (uint32_t u = tid; u < U && u < kCursorCap; u += kThreads)
+    cur[u] = (uint32_t)kids[u].start + row[kids[u].bucket];
+  __syncthreads();
+  BucketFn<KT> bk;
+  bk.init(st);
+  auto child_of = [&](KT raw) -> uint32_t { return amap[bk(to_key<C>(raw), lut)]; };
+  auto place = [&](uint32_t u, KT raw) {
+    uint32_t pos = u < kCursorCap
+                       ? atomicAdd(&cur[u], 1u)
+                       : (uint32_t)kids[u].start + atomicAdd(&row[kids[u].bucket], 1u);
     red[pos] = raw;
   };
   const KT* p = src + sd.data_off + ch.start;
@@ -718,13 +762,13 @@ __global__ __launch_bounds__(kThreads) void k_compact(const KT* __restrict__ src
   // own one of each)
   if (tid < head) {
     const KT raw = p[tid];
-    uint32_t b, u = child_of(raw, b);
-    if (u != ~0u) place(u, b, raw);
+    const uint32_t u = child_of(raw);
+    if (u != kNoChild) place(u, raw);
   }
   if (done + tid < len) {
     const KT raw = p[done + tid];
-    uint32_t b, u = child_of(raw, b);
-    if (u != ~0u) place(u, b, raw);
+    const uint32_t u = child_of(raw);
+    if (u != kNoChild) place(u, raw);
   }
   const uint4* __restrict__ pv = reinterpret_cast<const uint4*>(p + head);
   KT* wv = sval + wave * kStageCap;
@@ -734,29 +778,41 @@ __global__ __launch_bounds__(kThreads) void k_compact(const KT* __restrict__ src
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    for (uint32_t i = lane; i < cnt; i += 64) {
-      uint32_t u = wu[i];
-      KT raw = wv[i];
-      uint32_t b = 0;
-      if (!lds_cur) b = kids[u].bucket;
-      place(u, b, raw);
-    }
+    for (uint32_t i = lane; i < cnt; i += 64) place(wu[i], wv[i]);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     cnt = 0;
   };
-  auto stage = [&](KT raw, bool valid) {
-    uint32_t b, u = child_of(raw, b);
-    const bool act = valid && u != ~0u;
-    const uint64_t m = __ballot(act);
-    if (act) {
-      const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
-                                                       __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-      wv[cnt + below] = raw;
-      wu[cnt + below] = (uint16_t)u;
+  // stage the active keys among one 16-byte group per lane
+  auto group = [&](const uint4& v, bool ok) {
+    KT k[VEC];
+    if (sizeof(KT) == 4) {
+      k[0] = (KT)v.x; k[1] = (KT)v.y; k[2] = (KT)v.z; k[3] = (KT)v.w;
+    } else {
+      k[0] = (KT)(((uint64_t)v.y << 32) | v.x);
+      k[VEC - 1] = (KT)(((uint64_t)v.w << 32) | v.z);
     }
-    cnt += (uint32_t)__popcll(m);
+    uint32_t u[VEC], mask = 0;
+#pragma unroll
+    for (uint32_t e = 0; e < VEC; ++e) {
+      u[e] = child_of(k[e]);
+      mask |= (ok && u[e] != kNoChild) ? (1u << e) : 0u;
+    }
+    const uint32_t c = __popc(mask);
+    const uint32_t inc = wave_incl_scan(c);
+    const uint32_t tot = wave_last(inc);
+    if (tot == 0) return;  // uniform
+    uint32_t slot = cnt + inc - c;
+#pragma unroll
+    for (uint32_t e = 0; e < VEC; ++e) {
+      if ((mask >> e) & 1u) {
+        wv[slot] = k[e];
+        wu[slot] = (uint16_t)u[e];
+        ++slot;
+      }
+    }
+    cnt += tot;
   };
   const uint64_t per_round = 4ull * kThreads;
   const uint64_t rounds = (nvec + per_round - 1) / per_round;
@@ -773,13 +829,7 @@ __global__ __launch_bounds__(kThreads) void k_compact(const KT* __restrict__ src
     for (int t = 0; t < 4; ++t) {
       // a 16-byte group stages at most VEC * 64 keys per wavefront
       if (cnt > kStageCap - VEC * 64) flush();
-      if (sizeof(KT) == 4) {
-        stage((KT)v[t].x, ok[t]); stage((KT)v[t].y, ok[t]);
-        stage((KT)v[t].z, ok[t]); stage((KT)v[t].w, ok[t]);
-      } else {
-        stage((KT)(((uint64_t)v[t].y << 32) | v[t].x), ok[t]);
-        stage((KT)(((uint64_t)v[t].w << 32) | v[t].z), ok[t]);
-      }
+      group(v[t], ok[t]);
     }
   }
   if (cnt) flush();

@@ -14,7 +14,8 @@ namespace ggks {
 constexpr int kThreads = 1024;             // work-group size of every kernel
 constexpr uint32_t kMaxBuckets = 32768;    // histogram bins per segment (LDS)
 constexpr uint32_t kMaxPrefixBits = 12;    // LUT prefixes per segment <= 4096
-constexpr uint32_t kCursorCap = 8192;      // LDS write cursors in compaction
+constexpr uint32_t kCursorCap = 4096;      // LDS write cursors in compaction
+constexpr uint16_t kNoChild = 0xFFFF;      // child map: bucket not active
 constexpr uint32_t kFinishLdsBytes = 65536;// keys kept in LDS by the finisher
 constexpr uint32_t kQCap = 1024;           // queries per finisher batch
 constexpr uint32_t kGroupMax = 32;         // prefix groups per histogram pass
@@ -57,6 +58,7 @@ struct SegDesc {          // planned on the host, read-only on the device
   uint32_t lut_off;       // first LUT entry (u32)
   uint32_t tot_off;       // first total (u32), bmax entries
   uint32_t bits_off;      // first bitmap word (u32), bmax/32 words
+  uint32_t amap_off;      // first child-map entry (u16), bmax entries
   uint32_t q_off, q_cnt;  // its queries; child slots use the same range
   uint32_t mode;          // SegMode
   uint32_t radix_bits;    // MODE_RADIX digit width
@@ -64,14 +66,23 @@ struct SegDesc {          // planned on the host, read-only on the device
   uint32_t out_off;       // first output slot (QSRC_K, QSRC_TOPK)
 };
 
+// The bucket function of a segment (BucketFn in ggks_kernels.hpp):
+//   low = min(max(key, kb) - kb, lmax)      keys outside the sampled range
+//                                           fall into the edge buckets
+//   p   = (low >> poff) & (2^pwid - 1)      prefix, looked up in the LUT
+//   t   = ((low >> rsh) & (2^twid - 1)) << lsh   the low s bits of low,
+//                                           normalised to 24 bits
+//   b   = base[p] + (t * nb[p]) >> 24       (one 24-bit multiply-high)
 struct SegState {         // written on the device by the LUT kernel
-  uint64_t kb;            // smallest key of the interior buckets
-  uint64_t kmax;          // largest sampled key
-  uint32_t s;             // prefix shift: prefix = (key - kb) >> s
+  uint64_t kb;            // base key
+  uint64_t kmax;          // largest sampled key (information)
+  uint64_t lmax;          // clamp of key - kb
+  uint32_t s;             // prefix shift
   uint32_t P;             // number of prefixes
-  uint32_t B;             // buckets including tails
-  uint32_t Bi;            // interior buckets
+  uint32_t B;             // number of buckets
   uint32_t nact;          // active buckets (children)
+  uint32_t poff, pwid;    // prefix bit field
+  uint32_t rsh, twid, lsh;// sub-bucket bit field and normalising shift
   uint32_t pad;
 };
 
