@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--dtype", default="f32", choices=["f32", "f64", "u32", "i32"])
     ap.add_argument("--numk", type=int, default=1024)
     ap.add_argument("--kind", default="bucket", choices=["bucket", "radix"])
+    ap.add_argument("--blocks", type=int, default=0, help="chunk count (0 = tuned)")
+    ap.add_argument("--buckets-log2", type=int, default=0, help="bucket budget (0 = tuned)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     return ap.parse_args()
@@ -126,8 +128,9 @@ def main():
     # correctness of the timed configuration against the sync path's answer
     # on the same data is covered by the tests; here the outputs of the first
     # and last timed step must agree
+    opt = {"blocks": a.blocks, "buckets_log2": a.buckets_log2}
     for _ in range(a.warmup):
-        pkg.multiselect_async(d, dk, dout, ctx, kind=a.kind)
+        pkg.multiselect_async(d, dk, dout, ctx, kind=a.kind, **opt)
     L.check(lib.ggks_sync())
     first = dout.to_host()
     per = L.NSTAGES + 1
@@ -135,7 +138,7 @@ def main():
     L.check(lib.ggks_sync())
     t0 = time.perf_counter()
     for _ in range(a.steps):
-        pkg.multiselect_async(d, dk, dout, ctx, kind=a.kind)
+        pkg.multiselect_async(d, dk, dout, ctx, kind=a.kind, **opt)
     L.check(lib.ggks_sync())
     wall = time.perf_counter() - t0
     ms = np.zeros(a.steps * per, np.float32)
@@ -179,7 +182,7 @@ def main():
             "workload": f"{a.kind} multiselect, n=2^{a.log2n} {a.dtype} "
                         f"{'N(0,1)' if a.dtype in ('f32', 'f64') else 'uniform'}, "
                         f"{k.size} uniform k (sorted, deduplicated)",
-            "n": n, "num_k": int(k.size),
+            "n": n, "num_k": int(k.size), "blocks": a.blocks, "buckets_log2": a.buckets_log2,
             "hbm_frac_call": round(call_frac, 4),
             "alg_bytes_per_step": 2 * n * esz,
             "wall_ms_per_step": round(wall_ms, 4),

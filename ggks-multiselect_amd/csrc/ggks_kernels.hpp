@@ -159,11 +159,13 @@ struct BucketFn {
     rsh = st.rsh;
     twid = st.twid;
     lsh = st.lsh;
+    sc_off = st.sc_off;
+    sc_wid = st.sc_wid;
   }
-  __device__ __forceinline__ uint32_t operator()(KT key, const uint32_t* lut) const {
+  // stage 1: LUT index p and the normalised sub-field t of a key
+  __device__ __forceinline__ void split(KT key, uint32_t& p, uint32_t& t) const {
     KT low = (key > kb ? key : kb) - kb;
     low = low < lmax ? low : lmax;
-    uint32_t p, t;
     if (sizeof(KT) == 4) {
       p = __builtin_amdgcn_ubfe((uint32_t)low, poff, pwid);
       t = __builtin_amdgcn_ubfe((uint32_t)low, rsh, twid) << lsh;
@@ -172,12 +174,72 @@ struct BucketFn {
       p = (uint32_t)(l64 >> poff) & ((1u << pwid) - 1u);
       t = ((uint32_t)(l64 >> rsh) & ((1u << twid) - 1u)) << lsh;
     }
-    const uint32_t e = lut[p];
+  }
+  // stage 2: bucket from the LUT entry e = lut[p]
+  __device__ __forceinline__ uint32_t finish(uint32_t e, uint32_t t) const {
     const uint32_t nbsh = (e >> 16) << 8;  // nb * 2^8 < 2^24
     const uint32_t j = (uint32_t)(((uint64_t)(t & 0xFFFFFFu) * (nbsh & 0xFFFFFFu)) >> 32);
     return (e & 0xFFFFu) + j;
   }
+  __device__ __forceinline__ uint32_t operator()(KT key, const uint32_t* lut) const {
+    uint32_t p, t;
+    split(key, p, t);
+    return finish(lut[p], t);
+  }
+  // candidate cell: prefix and the top sc_wid bits of the sub-field (no LUT)
+  uint32_t sc_off, sc_wid;
+  __device__ __forceinline__ uint32_t cell(KT key) const {
+    KT low = (key > kb ? key : kb) - kb;
+    low = low < lmax ? low : lmax;
+    uint32_t p, c;
+    if (sizeof(KT) == 4) {
+      p = __builtin_amdgcn_ubfe((uint32_t)low, poff, pwid);
+      c = __builtin_amdgcn_ubfe((uint32_t)low, sc_off, sc_wid);
+    } else {
+      const uint64_t l64 = (uint64_t)low;
+      p = (uint32_t)(l64 >> poff) & ((1u << pwid) - 1u);
+      c = (uint32_t)(l64 >> sc_off) & ((1u << sc_wid) - 1u);
+    }
+    return (p << kCellBits) | c;
+  }
 };
+
+// Buckets of N keys with the N look-ups issued back to back, so a wavefront
+// waits on LDS once per batch rather than once per key.
+template <typename KT, int C, int N>
+__device__ __forceinline__ void bucket_batch(const BucketFn<KT>& fn, const uint32_t* lut,
+                                             const KT (&raw)[N], uint32_t (&b)[N]) {
+  uint32_t p[N], t[N], e[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) fn.split(to_key<C>(raw[i]), p[i], t[i]);
+#pragma unroll
+  for (int i = 0; i < N; ++i) e[i] = lut[p[i]];
+#pragma unroll
+  for (int i = 0; i < N; ++i) b[i] = fn.finish(e[i], t[i]);
+}
+
+// Streaming 16-byte load of input data: non-temporal, so the read stream
+// does not displace lines the kernel reuses (LUT, histogram rows, outputs).
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ uint4 stream_load(const uint4* p) {
+  const u32x4_t v = __builtin_nontemporal_load(reinterpret_cast<const u32x4_t*>(p));
+  return make_uint4(v.x, v.y, v.z, v.w);
+}
+
+// Unpack R 16-byte vectors into their keys.
+template <typename KT, int R>
+__device__ __forceinline__ void unpack(const uint4 (&v)[R], KT (&k)[R * 16 / sizeof(KT)]) {
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    if (sizeof(KT) == 4) {
+      k[r * 4 + 0] = (KT)v[r].x; k[r * 4 + 1] = (KT)v[r].y;
+      k[r * 4 + 2] = (KT)v[r].z; k[r * 4 + 3] = (KT)v[r].w;
+    } else {
+      k[r * 2 + 0] = (KT)(((uint64_t)v[r].y << 32) | v[r].x);
+      k[r * 2 + 1] = (KT)(((uint64_t)v[r].w << 32) | v[r].z);
+    }
+  }
+}
 
 // Key range [lo, hi] of a bucket, found by binary search over keys with the
 // same device function the passes use, so it can never disagree with them.
@@ -431,6 +493,8 @@ __global__ __launch_bounds__(kThreads) void k_lut(const SegDesc* __restrict__ se
     st.twid = s < 24 ? s : 24;
     st.rsh = s > 24 ? s - 24 : 0;
     st.lsh = s < 24 ? 24 - s : 0;
+    st.sc_wid = s < kCellBits ? s : kCellBits;
+    st.sc_off = s - st.sc_wid;
     states[blockIdx.x] = st;
   }
 }
@@ -455,9 +519,41 @@ __global__ __launch_bounds__(kThreads) void k_count(const KT* __restrict__ src,
   __syncthreads();
   BucketFn<KT> bk;
   bk.init(st);
-  for_each_elem<KT>(src + sd.data_off + ch.start, ch.len, [&](KT raw) {
-    atomicAdd(&hist[bk(to_key<C>(raw), lut)], 1u);
-  });
+  constexpr uint32_t VEC = 16 / sizeof(KT);
+  constexpr int R = 4, N = R * VEC;
+  const KT* p = src + sd.data_off + ch.start;
+  const uint64_t len = ch.len;
+  const uint64_t mis = ((uintptr_t)p & 15u) / sizeof(KT);
+  uint64_t head = mis ? (VEC - mis) : 0;
+  if (head > len) head = len;
+  const uint64_t nvec = (len - head) / VEC;
+  const uint64_t done = head + nvec * VEC;
+  const uint32_t tid = threadIdx.x;
+  if (tid < head) atomicAdd(&hist[bk(to_key<C>(p[tid]), lut)], 1u);
+  if (done + tid < len) atomicAdd(&hist[bk(to_key<C>(p[done + tid]), lut)], 1u);
+  const uint4* __restrict__ pv = reinterpret_cast<const uint4*>(p + head);
+  const uint64_t per_round = (uint64_t)R * kThreads;
+  uint64_t r0 = 0;
+  for (; r0 + per_round <= nvec; r0 += per_round) {  // full rounds
+    uint4 v[R];
+#pragma unroll
+    for (int t = 0; t < R; ++t) v[t] = stream_load(&pv[r0 + (uint64_t)t * kThreads + tid]);
+    KT k[N];
+    unpack<KT, R>(v, k);
+    uint32_t b[N];
+    bucket_batch<KT, C, N>(bk, lut, k, b);
+#pragma unroll
+    for (int i = 0; i < N; ++i) atomicAdd(&hist[b[i]], 1u);
+  }
+  for (uint64_t i = r0 + tid; i < nvec; i += kThreads) {  // last partial round
+    uint4 v[1] = {stream_load(&pv[i])};
+    KT k[VEC];
+    unpack<KT, 1>(v, k);
+    uint32_t b[VEC];
+    bucket_batch<KT, C, VEC>(bk, lut, k, b);
+#pragma unroll
+    for (uint32_t j = 0; j < VEC; ++j) atomicAdd(&hist[b[j]], 1u);
+  }
   __syncthreads();
   uint32_t* row = rows + sd.row_base + (uint64_t)ch.idx * bmax;
   for (uint32_t i = threadIdx.x; i < st.B; i += kThreads) row[i] = hist[i];
@@ -548,7 +644,9 @@ __global__ __launch_bounds__(kThreads) void k_findk(const SegDesc* __restrict__ 
                                                     uint32_t* __restrict__ qr,
                                                     Child* __restrict__ children,
                                                     Query* __restrict__ cq,
-                                                    uint16_t* __restrict__ amap_pool,
+                                                    uint32_t* __restrict__ bits_pool,
+                                                    uint32_t* __restrict__ wbase_pool,
+                                                    uint32_t* __restrict__ cand_pool,
                                                     KT* __restrict__ out) {
   extern __shared__ uint32_t lds[];
   const SegDesc sd = segs[blockIdx.x];
@@ -568,6 +666,9 @@ __global__ __launch_bounds__(kThreads) void k_findk(const SegDesc* __restrict__ 
   strided_for(totals + sd.tot_off, B, [&](uint32_t i, uint32_t v) { hist[i] = v; });
   strided_for(lut_pool + sd.lut_off, st.P, [&](uint32_t i, uint32_t v) { lut[i] = v; });
   for (uint32_t i = tid; i < nwords; i += kThreads) bits[i] = 0;
+  uint32_t* cand = cand_pool + sd.cand_off;
+  const uint32_t cwords = st.P << (kCellBits - 5);
+  for (uint32_t i = tid; i < cwords; i += kThreads) cand[i] = 0;
   __syncthreads();
   const uint32_t len = block_scan_array(hist, B, wsum);  // hist = exclusive
   RangeOf<KT> range;
@@ -615,12 +716,11 @@ __global__ __launch_bounds__(kThreads) void k_findk(const SegDesc* __restrict__ 
     wb[tid] = ubase;
     wc[tid] = cbase;
   }
-  __syncthreads();
-  for (uint32_t b = tid; b < B; b += kThreads) {  // consecutive threads, consecutive slots
-    const uint32_t mw = bits[b >> 5], sh = b & 31u;
-    amap_pool[sd.amap_off + b] =
-        ((mw >> sh) & 1u) ? (uint16_t)(wb[b >> 5] + __popc(mw & ((1u << sh) - 1u))) : kNoChild;
+  if (tid < nwords) {
+    bits_pool[sd.bits_off + tid] = bits[tid];
+    wbase_pool[sd.bits_off + tid] = ubase;
   }
+  __syncthreads();
   // one thread per child: its bucket, reduced slice and key range
   for (uint32_t u = tid; u < U; u += kThreads) {
     uint32_t a = 0, z = nwords - 1;  // largest word with wb <= u
@@ -648,6 +748,14 @@ __global__ __launch_bounds__(kThreads) void k_findk(const SegDesc* __restrict__ 
     c.bucket = b;
     c.seg = blockIdx.x;
     children[sd.q_off + u] = c;
+    // mark every candidate cell the bucket's keys can fall in
+    const uint32_t c0 = range.fn.cell(klo), c1 = range.fn.cell(khi);
+    for (uint32_t w = c0 >> 5; w <= (c1 >> 5); ++w) {
+      const uint32_t lo_b = w == (c0 >> 5) ? (c0 & 31u) : 0u;
+      const uint32_t hi_b = w == (c1 >> 5) ? (c1 & 31u) : 31u;
+      const uint32_t m = (hi_b == 31u ? 0xFFFFFFFFu : ((2u << hi_b) - 1u)) & ~((1u << lo_b) - 1u);
+      atomicOr(&cand[w], m);
+    }
   }
   __syncthreads();
 
@@ -688,22 +796,26 @@ __global__ __launch_bounds__(kThreads) void k_findk(const SegDesc* __restrict__ 
   if (tid == 0) states[blockIdx.x].nact = U;
 }
 
-// Pass 2: copy the keys of active buckets into their children's ranges.
-//
-// A thread maps its keys to children through the LDS child map.  Active keys
-// are staged per wavefront in LDS (slot = wavefront prefix sum of per-lane
-// counts, no atomics) and flushed in batches: one LDS cursor atomic per key,
-// 64 lanes at a time, then the global store.  Rounds are uniform so every lane
-// of a wavefront takes part in every shuffle.
+// Pass 2 (k_compact): stream the chunk once and move the keys of active
+// buckets into their children's ranges (each chunk owns a private slice of
+// every child: its running offsets start at the chunk's row entries).  The
+// first filter per key is its candidate cell (prefix plus the top bits of its
+// sub-field, no LUT look-up and no multiply) tested in an LDS bitmap that
+// marks every cell overlapping an active bucket.  Per round a lane spills its
+// keys to LDS and lists the indices of its candidates (slots from one
+// wavefront scan); the wavefront then settles the listed candidates 64 at a
+// time: exact bucket, active test, LDS cursor, store.
 constexpr uint32_t kWaves = kThreads / 64;
-// staged keys per wavefront: two 16-byte groups of the whole wavefront
-template <typename KT>
-constexpr uint32_t stage_cap() { return 2 * 64 * (16 / sizeof(KT)); }
+constexpr int kRound = 4;                   // 16-byte loads per lane per round
+constexpr uint32_t kCompactCursors = 1024;  // LDS cursors; children beyond use global atomics
 
 template <typename KT>
 __host__ __device__ constexpr size_t compact_lds_bytes(uint32_t pbits, uint32_t bmax) {
-  return (size_t)(1u << pbits) * 4 + (size_t)bmax * 2 + (size_t)kCursorCap * 4 +
-         (size_t)kWaves * stage_cap<KT>() * (sizeof(KT) + 2);
+  // candidate bitmap, LUT, (bits, child base) per word, cursors, the round's
+  // keys of every lane (64 B each) and the staged candidate indices
+  return ((size_t)1 << (pbits + kCellBits - 3)) + (size_t)(1u << pbits) * 4 +
+         (size_t)((bmax + 31) / 32) * 8 + (size_t)kCompactCursors * 4 +
+         (size_t)kThreads * kRound * 16 + (size_t)kThreads * kRound * (16 / sizeof(KT)) * 2;
 }
 
 template <typename KT, int C>
@@ -713,7 +825,9 @@ __global__ __launch_bounds__(kThreads) void k_compact(const KT* __restrict__ src
                                                       const Chunk* __restrict__ chunks,
                                                       const uint32_t* __restrict__ lut_pool,
                                                       uint32_t* __restrict__ rows,
-                                                      const uint16_t* __restrict__ amap_pool,
+                                                      const uint32_t* __restrict__ bits_pool,
+                                                      const uint32_t* __restrict__ wbase_pool,
+                                                      const uint32_t* __restrict__ cand_pool,
                                                       const Child* __restrict__ children,
                                                       KT* __restrict__ red) {
   extern __shared__ __align__(16) uint32_t lds[];
@@ -722,117 +836,99 @@ __global__ __launch_bounds__(kThreads) void k_compact(const KT* __restrict__ src
   const SegState st = states[ch.seg];
   const uint32_t U = st.nact;
   if (U == 0) return;
+  constexpr uint32_t VEC = 16 / sizeof(KT);
+  constexpr int N = kRound * VEC;  // keys per lane per round
+  const uint32_t nwords = (st.B + 31) / 32;
+  const uint32_t cwords = st.P << (kCellBits - 5);
   char* base = reinterpret_cast<char*>(lds);
-  uint32_t* lut = reinterpret_cast<uint32_t*>(base);
-  uint16_t* amap = reinterpret_cast<uint16_t*>(base + (size_t)(1u << sd.pbits) * 4);
-  uint32_t* cur = reinterpret_cast<uint32_t*>(amap + (sd.bmax + 1) / 2 * 2);
-  KT* sval = reinterpret_cast<KT*>(cur + kCursorCap);
-  constexpr uint32_t kStageCap = stage_cap<KT>();
-  uint16_t* su = reinterpret_cast<uint16_t*>(sval + kWaves * kStageCap);
+  uint32_t* cbm = reinterpret_cast<uint32_t*>(base);
+  uint32_t* lut = reinterpret_cast<uint32_t*>(base + ((size_t)1 << (sd.pbits + kCellBits - 3)));
+  uint2* bw = reinterpret_cast<uint2*>(lut + (1u << sd.pbits));  // (bits, child base)
+  uint32_t* cur = reinterpret_cast<uint32_t*>(bw + (sd.bmax + 31) / 32);
+  uint4* kbuf = reinterpret_cast<uint4*>(cur + kCompactCursors);  // kRound per lane
+  uint16_t* sidx = reinterpret_cast<uint16_t*>(kbuf + kThreads * kRound);
   const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = lane_id();
-  for (uint32_t i = tid; i < st.P; i += kThreads) lut[i] = lut_pool[sd.lut_off + i];
-  {
-    const uint32_t* a32 = reinterpret_cast<const uint32_t*>(amap_pool + sd.amap_off);
-    uint32_t* d32 = reinterpret_cast<uint32_t*>(amap);
-    strided_for(a32, (st.B + 1) / 2, [&](uint32_t i, uint32_t v) { d32[i] = v; });
-  }
+  strided_for(cand_pool + sd.cand_off, cwords, [&](uint32_t i, uint32_t v) { cbm[i] = v; });
+  strided_for(lut_pool + sd.lut_off, st.P, [&](uint32_t i, uint32_t v) { lut[i] = v; });
+  for (uint32_t i = tid; i < nwords; i += kThreads)
+    bw[i] = make_uint2(bits_pool[sd.bits_off + i], wbase_pool[sd.bits_off + i]);
   uint32_t* row = rows + sd.row_base + (uint64_t)ch.idx * sd.bmax;
   const Child* kids = children + sd.q_off;
-  for (uint32_t u = tid; u < U && u < kCursorCap; u += kThreads)
+  for (uint32_t u = tid; u < U && u < kCompactCursors; u += kThreads)
     cur[u] = (uint32_t)kids[u].start + row[kids[u].bucket];
   __syncthreads();
   BucketFn<KT> bk;
   bk.init(st);
-  auto child_of = [&](KT raw) -> uint32_t { return amap[bk(to_key<C>(raw), lut)]; };
-  auto place = [&](uint32_t u, KT raw) {
-    uint32_t pos = u < kCursorCap
-                       ? atomicAdd(&cur[u], 1u)
-                       : (uint32_t)kids[u].start + atomicAdd(&row[kids[u].bucket], 1u);
-    red[pos] = raw;
+  // exact test of a candidate; stores it into its child's range if active
+  auto settle = [&](KT raw) {
+    const uint32_t b = bk(to_key<C>(raw), lut);
+    const uint2 e = bw[b >> 5];
+    const uint32_t sh = b & 31u;
+    if ((e.x >> sh) & 1u) {
+      const uint32_t u = e.y + __popc(e.x & ((1u << sh) - 1u));
+      const uint32_t pos = u < kCompactCursors
+                               ? atomicAdd(&cur[u], 1u)
+                               : (uint32_t)kids[u].start + atomicAdd(&row[kids[u].bucket], 1u);
+      red[pos] = raw;
+    }
   };
   const KT* p = src + sd.data_off + ch.start;
   const uint64_t len = ch.len;
-  constexpr uint32_t VEC = 16 / sizeof(KT);
   uint64_t mis = ((uintptr_t)p & 15u) / sizeof(KT);
   uint64_t head = mis ? (VEC - mis) : 0;
   if (head > len) head = len;
   const uint64_t nvec = (len - head) / VEC;
   const uint64_t done = head + nvec * VEC;
-  // head and tail: fewer than VEC keys each, placed directly (a thread may
-  // own one of each)
-  if (tid < head) {
-    const KT raw = p[tid];
-    const uint32_t u = child_of(raw);
-    if (u != kNoChild) place(u, raw);
-  }
-  if (done + tid < len) {
-    const KT raw = p[done + tid];
-    const uint32_t u = child_of(raw);
-    if (u != kNoChild) place(u, raw);
-  }
+  // head and tail: fewer than VEC keys each (a thread may own one of each)
+  if (tid < head) settle(p[tid]);
+  if (done + tid < len) settle(p[done + tid]);
   const uint4* __restrict__ pv = reinterpret_cast<const uint4*>(p + head);
-  KT* wv = sval + wave * kStageCap;
-  uint16_t* wu = su + wave * kStageCap;
-  uint32_t cnt = 0;  // staged keys of this wavefront (uniform)
-  auto flush = [&]() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    for (uint32_t i = lane; i < cnt; i += 64) place(wu[i], wv[i]);
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    cnt = 0;
-  };
-  // stage the active keys among one 16-byte group per lane
-  auto group = [&](const uint4& v, bool ok) {
-    KT k[VEC];
-    if (sizeof(KT) == 4) {
-      k[0] = (KT)v.x; k[1] = (KT)v.y; k[2] = (KT)v.z; k[3] = (KT)v.w;
-    } else {
-      k[0] = (KT)(((uint64_t)v.y << 32) | v.x);
-      k[VEC - 1] = (KT)(((uint64_t)v.w << 32) | v.z);
-    }
-    uint32_t u[VEC], mask = 0;
-#pragma unroll
-    for (uint32_t e = 0; e < VEC; ++e) {
-      u[e] = child_of(k[e]);
-      mask |= (ok && u[e] != kNoChild) ? (1u << e) : 0u;
-    }
-    const uint32_t c = __popc(mask);
-    const uint32_t inc = wave_incl_scan(c);
-    const uint32_t tot = wave_last(inc);
-    if (tot == 0) return;  // uniform
-    uint32_t slot = cnt + inc - c;
-#pragma unroll
-    for (uint32_t e = 0; e < VEC; ++e) {
-      if ((mask >> e) & 1u) {
-        wv[slot] = k[e];
-        wu[slot] = (uint16_t)u[e];
-        ++slot;
-      }
-    }
-    cnt += tot;
-  };
-  const uint64_t per_round = 4ull * kThreads;
+  // lane-interleaved spill: wavefront w, load t, lane l at kbuf[(w*kRound + t)*64 + l],
+  // so a 16-byte LDS write of the whole wavefront is one contiguous KiB
+  uint4* my_keys = kbuf + (size_t)wave * 64 * kRound + lane;
+  const KT* wave_keys = reinterpret_cast<const KT*>(kbuf + (size_t)wave * 64 * kRound);
+  uint16_t* wave_idx = sidx + (size_t)wave * 64 * N;
+  const uint64_t per_round = (uint64_t)kRound * kThreads;
   const uint64_t rounds = (nvec + per_round - 1) / per_round;
   for (uint64_t r = 0; r < rounds; ++r) {
-    uint4 v[4];
-    bool ok[4];
+    uint4 v[kRound];
+    uint32_t okmask = 0;
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      uint64_t idx = r * per_round + (uint64_t)t * kThreads + tid;
-      ok[t] = idx < nvec;
-      v[t] = ok[t] ? pv[idx] : make_uint4(0, 0, 0, 0);
+    for (int t = 0; t < kRound; ++t) {
+      const uint64_t idx = r * per_round + (uint64_t)t * kThreads + tid;
+      const bool ok = idx < nvec;
+      okmask |= ok ? (((1u << VEC) - 1u) << (t * VEC)) : 0u;
+      v[t] = ok ? stream_load(&pv[idx]) : make_uint4(0, 0, 0, 0);
     }
+    KT k[N];
+    unpack<KT, kRound>(v, k);
+    uint32_t c[N], mask = 0;
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      // a 16-byte group stages at most VEC * 64 keys per wavefront
-      if (cnt > kStageCap - VEC * 64) flush();
-      group(v[t], ok[t]);
+    for (int i = 0; i < N; ++i) c[i] = bk.cell(to_key<C>(k[i]));
+#pragma unroll
+    for (int i = 0; i < N; ++i) mask |= ((cbm[c[i] >> 5] >> (c[i] & 31u)) & 1u) << i;
+    mask &= okmask;
+    const uint32_t cl = __popc(mask);
+    const uint32_t inc = wave_incl_scan(cl);
+    const uint32_t tot = wave_last(inc);
+    if (tot == 0) continue;  // uniform
+    // this lane's keys, then the indices of its candidates
+#pragma unroll
+    for (int t = 0; t < kRound; ++t) my_keys[t * 64] = v[t];
+    // key i of this lane sits in load t = i / VEC, component i % VEC
+    uint32_t slot = inc - cl;
+    for (uint32_t m = mask; m; m &= m - 1) {
+      const uint32_t i = __ffs(m) - 1;
+      wave_idx[slot++] = (uint16_t)(((i / VEC) * 64 + lane) * VEC + (i % VEC));
     }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (uint32_t j = lane; j < tot; j += 64) settle(wave_keys[wave_idx[j]]);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
-  if (cnt) flush();
 }
 
 // Finisher: exact multi-rank radix select inside each child.

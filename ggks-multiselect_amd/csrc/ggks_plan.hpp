@@ -48,7 +48,7 @@ struct Plan {
   // arena offsets (bytes)
   size_t off_tables = 0, off_segs = 0, off_chunks = 0, off_items = 0, tables_end = 0;
   size_t off_states = 0, off_samples = 0, off_lut = 0, off_rows = 0, off_tot = 0;
-  size_t off_amap = 0, off_q = 0, off_qb = 0, off_qr = 0;
+  size_t off_bits = 0, off_wbase = 0, off_cand = 0, off_q = 0, off_qb = 0, off_qr = 0;
   size_t off_children = 0, off_cq = 0, off_red = 0, total = 0;
 
   size_t table_bytes() const { return tables_end - off_tables; }
@@ -135,7 +135,7 @@ inline void plan_single(Plan& p, int kind, int dtype, uint64_t n, uint32_t nq,
   s.lut_off = 0;
   s.tot_off = 0;
   s.bits_off = 0;
-  s.amap_off = 0;
+  s.cand_off = 0;
   s.q_off = 0;
   s.q_cnt = nq;
   s.mode = mode;
@@ -167,7 +167,9 @@ inline void plan_single(Plan& p, int kind, int dtype, uint64_t n, uint32_t nq,
   p.off_lut = take((size_t)(1u << pb) * 4);
   p.off_rows = take((size_t)G * bmax * 4);
   p.off_tot = take((size_t)bmax * 4);
-  p.off_amap = take((size_t)bmax * 2);
+  p.off_bits = take((size_t)((bmax + 31) / 32) * 4);
+  p.off_wbase = take((size_t)((bmax + 31) / 32) * 4);
+  p.off_cand = take((size_t)(1u << pb) << (kCellBits - 3));
   p.off_q = take((size_t)nq * sizeof(Query));
   p.off_qb = take((size_t)nq * 4);
   p.off_qr = take((size_t)nq * 4);

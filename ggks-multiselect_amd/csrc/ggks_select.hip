@@ -153,7 +153,9 @@ static int launch_level(ggks_ctx* c, const Plan& p, const KT* d_src, KT* d_out,
   uint32_t* lut = reinterpret_cast<uint32_t*>(A + p.off_lut);
   uint32_t* rows = reinterpret_cast<uint32_t*>(A + p.off_rows);
   uint32_t* tot = reinterpret_cast<uint32_t*>(A + p.off_tot);
-  uint16_t* amap = reinterpret_cast<uint16_t*>(A + p.off_amap);
+  uint32_t* bits = reinterpret_cast<uint32_t*>(A + p.off_bits);
+  uint32_t* wbase = reinterpret_cast<uint32_t*>(A + p.off_wbase);
+  uint32_t* cand = reinterpret_cast<uint32_t*>(A + p.off_cand);
   const Query* q = reinterpret_cast<const Query*>(A + p.off_q);
   uint32_t* qb = reinterpret_cast<uint32_t*>(A + p.off_qb);
   uint32_t* qr = reinterpret_cast<uint32_t*>(A + p.off_qr);
@@ -184,12 +186,13 @@ static int launch_level(ggks_ctx* c, const Plan& p, const KT* d_src, KT* d_out,
       ((size_t)p.bmax_max + (1u << p.pbits_max) + 3 * ((p.bmax_max + 31) / 32) + 32) * 4;
   set_lds_limit(k_findk<KT, C>, lds_findk);
   hipLaunchKernelGGL((k_findk<KT, C>), dim3(nseg), dim3(kThreads), lds_findk, s, segs, states,
-                     lut, tot, q, d_k, c->status, qb, qr, kids, cq, amap, d_out);
+                     lut, tot, q, d_k, c->status, qb, qr, kids, cq, bits, wbase, cand, d_out);
   const size_t lds_compact = compact_lds_bytes<KT>(p.pbits_max, p.bmax_max);
   set_lds_limit(k_compact<KT, C>, lds_compact);
   mark(c, 6);
   hipLaunchKernelGGL((k_compact<KT, C>), dim3((uint32_t)p.chunks.size()), dim3(kThreads),
-                     lds_compact, s, d_src, segs, states, chunks, lut, rows, amap, kids, red);
+                     lds_compact, s, d_src, segs, states, chunks, lut, rows, bits, wbase, cand,
+                     kids, red);
   set_lds_limit(k_finish<KT, C>, kFinishLdsTotal);
   mark(c, 7);
   const uint32_t fgrid = std::min<uint32_t>(p.nq, 2048);
@@ -206,7 +209,7 @@ static int launch_direct(ggks_ctx* c, const Plan& p, const KT* d_src, KT* d_out)
   const Child* kids = reinterpret_cast<const Child*>(A + p.off_children);
   const Query* q = reinterpret_cast<const Query*>(A + p.off_q);
   set_lds_limit(k_finish<KT, C>, kFinishLdsTotal);
-  for (int st = 1; st <= 7; ++st) mark(c, st);
+  for (int st = 1; st < GGKS_NSTAGES; ++st) mark(c, st);
   hipLaunchKernelGGL((k_finish<KT, C>), dim3(1), dim3(kThreads), kFinishLdsTotal, c->stream,
                      d_src, (const KT*)nullptr, kids, 1u, q, d_out);
   HIP_TRY(hipGetLastError());
