@@ -48,6 +48,9 @@ def parse():
     ap.add_argument("--buckets-log2", type=int, default=0, help="bucket budget (0 = tuned)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="after the timed steps, copy the vector once with the library's plain "
+                         "16-byte copy kernel (known-size reference for memory-traffic counters)")
     return ap.parse_args()
 
 
@@ -147,6 +150,10 @@ def main():
     last = dout.to_host()
     consistent = bool(np.array_equal(first.view(np.uint8), last.view(np.uint8)))
     L.check(lib.ggks_ctx_profile_arm(ctx.handle, 0))
+    if a.copy_probe:
+        scratch = pkg.DeviceArray(n, x.dtype)
+        L.check(lib.ggks_d2d(scratch.ptr, d.ptr, n * esz))
+        del scratch
 
     step_ms = float(ms[:, L.NSTAGES].mean())
     wall_ms = wall * 1e3 / a.steps

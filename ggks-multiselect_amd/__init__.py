@@ -5,10 +5,10 @@ The directory name holds a dash, so it is imported by path:
 ``load_package()`` in __graft_entry__.py registers it as
 ``ggks_multiselect_amd``.
 """
-from .api import (Context, DeviceArray, DeviceView, GGKSError,  # noqa: F401
-                  RadixSelectWrapper, bucketMultiselectWrapper, multiselect,
+from .api import (Batch, Context, DeviceArray, DeviceView, GGKSError,  # noqa: F401
+                  RadixSelectWrapper, bucketMultiselectWrapper, generate, multiselect,
                   multiselect_async, randomizedTopkSelectWrapper, scratch_bytes,
-                  sync, topk)
+                  segment_table, sync, topk, tuning_get, tuning_reset, tuning_set)
 from . import workloads  # noqa: F401
 
 __version__ = "0.1"

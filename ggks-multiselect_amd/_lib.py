@@ -8,6 +8,8 @@ from __future__ import annotations
 import ctypes
 import os
 
+import numpy as np
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libggks.so")
 
@@ -26,8 +28,12 @@ EXPORTS = [
     "ggks_ctx_reserve", "ggks_ctx_check", "ggks_ctx_stream", "ggks_dev_alloc",
     "ggks_dev_free", "ggks_h2d", "ggks_d2h", "ggks_d2d", "ggks_sync",
     "ggks_device_count", "ggks_set_device", "ggks_dev_fill", "ggks_ctx_profile_arm",
-    "ggks_ctx_profile_read",
+    "ggks_ctx_profile_read", "ggks_batch_create", "ggks_batch_scratch_bytes", "ggks_batch_run",
+    "ggks_batch_sync", "ggks_batch_stream", "ggks_batch_info", "ggks_batch_destroy",
+    "ggks_tuning_set", "ggks_tuning_get", "ggks_tuning_reset", "ggks_generate",
 ]
+BATCH_GRAPH = 1
+GEN_UNIFORM, GEN_NORMAL = 0, 1
 NSTAGES = 8
 STAGES = ["queries", "sample", "lut", "count", "reduce", "findk", "compact", "finish"]
 
@@ -44,6 +50,26 @@ class Opts(ctypes.Structure):
                 ("buckets_log2", ctypes.c_int32), ("destructive", ctypes.c_int32),
                 ("seed", ctypes.c_uint64), ("radix_bits", ctypes.c_int32),
                 ("reserved", ctypes.c_int32 * 5)]
+
+
+class Segment(ctypes.Structure):
+    """ggks_segment (include/ggks.h)."""
+    _fields_ = [("offset", ctypes.c_uint64), ("length", ctypes.c_uint64),
+                ("k_offset", ctypes.c_uint64), ("out_offset", ctypes.c_uint64),
+                ("num_k", ctypes.c_uint32), ("kind", ctypes.c_uint32)]
+
+
+SEGMENT_DTYPE = np.dtype([("offset", "<u8"), ("length", "<u8"), ("k_offset", "<u8"),
+                          ("out_offset", "<u8"), ("num_k", "<u4"), ("kind", "<u4")])
+assert SEGMENT_DTYPE.itemsize == ctypes.sizeof(Segment)
+
+
+class BatchStats(ctypes.Structure):
+    """ggks_batch_stats (include/ggks.h)."""
+    _fields_ = [("elements", ctypes.c_uint64), ("queries", ctypes.c_uint64),
+                ("scratch_bytes", ctypes.c_uint64), ("engine_segments", ctypes.c_uint32),
+                ("direct_segments", ctypes.c_uint32), ("chunks", ctypes.c_uint32),
+                ("graph", ctypes.c_uint32)]
 
 
 _lib = None
@@ -86,6 +112,20 @@ def lib() -> ctypes.CDLL:
         "ggks_dev_fill": ([vp, i32, sz], i32),
         "ggks_ctx_profile_arm": ([vp, i32], i32),
         "ggks_ctx_profile_read": ([vp, vp, i32], i32),
+        "ggks_batch_create": ([ctypes.POINTER(vp), i32, i32, vp, i64, vp, i64, optp, vp, sz, vp],
+                              i32),
+        "ggks_batch_scratch_bytes": ([i32, vp, i64, vp, i64, optp], sz),
+        "ggks_batch_run": ([vp, vp, vp, i32], i32),
+        "ggks_batch_sync": ([vp], i32),
+        "ggks_batch_stream": ([vp], vp),
+        "ggks_batch_info": ([vp, ctypes.POINTER(BatchStats)], i32),
+        "ggks_batch_destroy": ([vp], i32),
+        "ggks_tuning_set": ([i32, i32, i32, i32, i32], i32),
+        "ggks_tuning_get": ([i32, i32, ctypes.POINTER(i32), ctypes.POINTER(i32),
+                             ctypes.POINTER(i32)], i32),
+        "ggks_tuning_reset": ([], i32),
+        "ggks_generate": ([vp, i32, i32, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64,
+                           i64, i64], i32),
     }
     for t in ("f32", "f64", "u32", "i32"):
         sig[f"ggks_bucket_multiselect_{t}"] = ([vp, i64, vp, ctypes.c_int32, vp, optp], i32)

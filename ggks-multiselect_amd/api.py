@@ -202,6 +202,124 @@ def randomizedTopkSelectWrapper(lst, size: int, k: int) -> np.ndarray:
     return topk(vec, k)
 
 
+# ------------------------------------------------------------ batched select
+_KIND_CODE = {"bucket": _lib.KIND_BUCKET, "radix": _lib.KIND_RADIX, "topk": _lib.KIND_TOPK}
+
+
+def segment_table(lengths, kinds, num_k, offsets=None, k_offsets=None,
+                  out_offsets=None) -> np.ndarray:
+    """A ggks_segment table (numpy structured array).  Offsets default to
+    back-to-back segments, k lists and outputs."""
+    lengths = np.asarray(lengths, np.uint64)
+    num_k = np.asarray(num_k, np.uint32)
+    kinds = np.asarray([_KIND_CODE[k] if isinstance(k, str) else int(k) for k in kinds], np.uint32)
+    t = np.zeros(lengths.size, _lib.SEGMENT_DTYPE)
+    t["length"] = lengths
+    t["num_k"] = num_k
+    t["kind"] = kinds
+    t["offset"] = offsets if offsets is not None else np.concatenate(
+        [[0], np.cumsum(lengths)[:-1]]).astype(np.uint64)
+    nk_lists = np.where(kinds == _lib.KIND_TOPK, 0, num_k).astype(np.uint64)
+    t["k_offset"] = k_offsets if k_offsets is not None else np.concatenate(
+        [[0], np.cumsum(nk_lists)[:-1]]).astype(np.uint64)
+    t["out_offset"] = out_offsets if out_offsets is not None else np.concatenate(
+        [[0], np.cumsum(num_k.astype(np.uint64))[:-1]]).astype(np.uint64)
+    return t
+
+
+class Batch:
+    """A planned ggks_batch: many segments of one device vector selected by
+    one launch sequence (optionally a replayed hipGraph).  `table` is a
+    segment_table(); `k` the concatenated k lists of its multiselect
+    segments."""
+
+    def __init__(self, dtype, table: np.ndarray, k=None, device: int = 0, scratch=None,
+                 stream: int | None = None, **opt):
+        self.dtype = np.dtype(dtype)
+        self.table = np.ascontiguousarray(table, dtype=_lib.SEGMENT_DTYPE)
+        self.k = np.ascontiguousarray(np.zeros(0, np.uint32) if k is None else k, np.uint32)
+        o = _opts(**opt)
+        p = ctypes.c_void_p()
+        sp = scratch.ptr if scratch is not None else None
+        sb = scratch.nbytes if scratch is not None else 0
+        check(lib().ggks_batch_create(ctypes.byref(p), int(device), dtype_code(self.dtype),
+                                      self.table.ctypes.data, self.table.size,
+                                      self.k.ctypes.data, self.k.size, ctypes.byref(o), sp, sb,
+                                      stream))
+        self.handle = p.value
+        self.num_outputs = int((self.table["out_offset"] + self.table["num_k"]).max()) \
+            if self.table.size else 0
+
+    @staticmethod
+    def scratch_bytes(dtype, table: np.ndarray, k=None, **opt) -> int:
+        table = np.ascontiguousarray(table, dtype=_lib.SEGMENT_DTYPE)
+        k = np.ascontiguousarray(np.zeros(0, np.uint32) if k is None else k, np.uint32)
+        o = _opts(**opt)
+        return int(lib().ggks_batch_scratch_bytes(dtype_code(dtype), table.ctypes.data,
+                                                  table.size, k.ctypes.data, k.size,
+                                                  ctypes.byref(o)))
+
+    def run(self, dvec, dout, graph: bool = False) -> None:
+        check(lib().ggks_batch_run(self.handle, dvec.ptr, dout.ptr,
+                                   _lib.BATCH_GRAPH if graph else 0))
+
+    def sync(self) -> None:
+        check(lib().ggks_batch_sync(self.handle))
+
+    def stats(self) -> dict:
+        st = _lib.BatchStats()
+        check(lib().ggks_batch_info(self.handle, ctypes.byref(st)))
+        return {f: int(getattr(st, f)) for f, _ in st._fields_}
+
+    def __call__(self, dvec, graph: bool = False) -> np.ndarray:
+        out = DeviceArray(max(self.num_outputs, 1), self.dtype)
+        self.run(dvec, out, graph)
+        self.sync()
+        return out.to_host()[:self.num_outputs]
+
+    def close(self):
+        if self.handle:
+            lib().ggks_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def tuning_set(dtype, log2_len: int, buckets_log2: int, chunk_log2: int, direct: bool) -> None:
+    check(lib().ggks_tuning_set(dtype_code(dtype), int(log2_len), int(buckets_log2),
+                                int(chunk_log2), int(bool(direct))))
+
+
+def tuning_get(dtype, log2_len: int) -> tuple:
+    b, c, d = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    check(lib().ggks_tuning_get(dtype_code(dtype), int(log2_len), ctypes.byref(b),
+                                ctypes.byref(c), ctypes.byref(d)))
+    return b.value, c.value, bool(d.value)
+
+
+def tuning_reset() -> None:
+    check(lib().ggks_tuning_reset())
+
+
+def generate(dtype, family: str, seed: int, n: int, start: int = 0, lo: int = 0, hi: int = 0,
+             out: DeviceArray | None = None, offset: int = 0):
+    """Fill a device array (new, or `out` from element `offset`) with n seeded
+    values: family "uniform" (floats in [0,1), integers in [lo, hi]) or
+    "normal"."""
+    d = out if out is not None else DeviceArray(n, dtype)
+    fam = {"uniform": _lib.GEN_UNIFORM, "normal": _lib.GEN_NORMAL}[family]
+    if offset < 0 or offset + n > d.n:
+        raise ValueError("generate range outside the array")
+    check(lib().ggks_generate(d.ptr + offset * d.dtype.itemsize, dtype_code(d.dtype), fam,
+                              int(seed) & 0xFFFFFFFFFFFFFFFF, int(start), int(n), int(lo), int(hi)))
+    return d
+
+
 __all__ = ["DeviceArray", "DeviceView", "Context", "GGKSError", "multiselect", "topk",
            "multiselect_async", "scratch_bytes", "sync", "bucketMultiselectWrapper",
-           "RadixSelectWrapper", "randomizedTopkSelectWrapper", "dtype_code"]
+           "RadixSelectWrapper", "randomizedTopkSelectWrapper", "dtype_code", "Batch",
+           "segment_table", "tuning_set", "tuning_get", "tuning_reset", "generate"]

@@ -890,9 +890,12 @@ __global__ __launch_bounds__(kThreads) void k_compact(const KT* __restrict__ src
   uint16_t* wave_idx = sidx + (size_t)wave * 64 * N;
   const uint64_t per_round = (uint64_t)kRound * kThreads;
   const uint64_t rounds = (nvec + per_round - 1) / per_round;
-  for (uint64_t r = 0; r < rounds; ++r) {
-    uint4 v[kRound];
-    uint32_t okmask = 0;
+  // software pipeline: the loads of round r + 1 are in flight while the
+  // candidates of round r settle
+  uint4 v[kRound];
+  uint32_t okmask = 0;
+  auto load_round = [&](uint64_t r) {
+    okmask = 0;
 #pragma unroll
     for (int t = 0; t < kRound; ++t) {
       const uint64_t idx = r * per_round + (uint64_t)t * kThreads + tid;
@@ -900,6 +903,9 @@ __global__ __launch_bounds__(kThreads) void k_compact(const KT* __restrict__ src
       okmask |= ok ? (((1u << VEC) - 1u) << (t * VEC)) : 0u;
       v[t] = ok ? stream_load(&pv[idx]) : make_uint4(0, 0, 0, 0);
     }
+  };
+  if (rounds) load_round(0);
+  for (uint64_t r = 0; r < rounds; ++r) {
     KT k[N];
     unpack<KT, kRound>(v, k);
     uint32_t c[N], mask = 0;
@@ -910,25 +916,37 @@ __global__ __launch_bounds__(kThreads) void k_compact(const KT* __restrict__ src
     mask &= okmask;
     const uint32_t cl = __popc(mask);
     const uint32_t inc = wave_incl_scan(cl);
-    const uint32_t tot = wave_last(inc);
-    if (tot == 0) continue;  // uniform
-    // this lane's keys, then the indices of its candidates
+    const uint32_t tot = wave_last(inc);  // uniform
+    if (tot) {
+      // this lane's keys, then the indices of its candidates
 #pragma unroll
-    for (int t = 0; t < kRound; ++t) my_keys[t * 64] = v[t];
-    // key i of this lane sits in load t = i / VEC, component i % VEC
-    uint32_t slot = inc - cl;
-    for (uint32_t m = mask; m; m &= m - 1) {
-      const uint32_t i = __ffs(m) - 1;
-      wave_idx[slot++] = (uint16_t)(((i / VEC) * 64 + lane) * VEC + (i % VEC));
+      for (int t = 0; t < kRound; ++t) my_keys[t * 64] = v[t];
+      // key i of this lane sits in load t = i / VEC, component i % VEC
+      uint32_t slot = inc - cl;
+      for (uint32_t m = mask; m; m &= m - 1) {
+        const uint32_t i = __ffs(m) - 1;
+        wave_idx[slot++] = (uint16_t)(((i / VEC) * 64 + lane) * VEC + (i % VEC));
+      }
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    for (uint32_t j = lane; j < tot; j += 64) settle(wave_keys[wave_idx[j]]);
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (r + 1 < rounds) load_round(r + 1);
+    if (tot) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      for (uint32_t j = lane; j < tot; j += 64) settle(wave_keys[wave_idx[j]]);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
   }
+}
+
+// Plain streaming copy of nv 16-byte words (grid-stride).
+__global__ __launch_bounds__(kThreads) void k_copy16(const uint4* __restrict__ src,
+                                                     uint4* __restrict__ dst, uint64_t nv) {
+  const uint64_t stride = (uint64_t)gridDim.x * kThreads;
+  for (uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x; i < nv; i += stride)
+    dst[i] = stream_load(&src[i]);
 }
 
 // Finisher: exact multi-rank radix select inside each child.

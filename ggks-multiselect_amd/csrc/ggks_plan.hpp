@@ -38,7 +38,7 @@ struct Plan {
   std::vector<Chunk> chunks;
   std::vector<ReduceItem> items;
   std::vector<Child> direct_children;
-  bool direct = false;
+  bool direct = false;  // single-vector plan answered by the finisher alone
   bool any_sample = false;
   uint32_t nq = 0;
   uint32_t max_nsample = 0;
@@ -50,6 +50,9 @@ struct Plan {
   size_t off_states = 0, off_samples = 0, off_lut = 0, off_rows = 0, off_tot = 0;
   size_t off_bits = 0, off_wbase = 0, off_cand = 0, off_q = 0, off_qb = 0, off_qr = 0;
   size_t off_children = 0, off_cq = 0, off_red = 0, total = 0;
+  size_t off_kvals = 0, off_dchildren = 0, off_dqueries = 0;  // batch / direct tables
+  std::vector<uint32_t> kvals;         // k lists of the engine segments, by q_off
+  std::vector<Query> direct_queries;   // explicit queries of the direct children
 
   size_t table_bytes() const { return tables_end - off_tables; }
 
@@ -59,9 +62,14 @@ struct Plan {
       memcpy(dst + (off_chunks - off_tables), chunks.data(), chunks.size() * sizeof(Chunk));
     if (!items.empty())
       memcpy(dst + (off_items - off_tables), items.data(), items.size() * sizeof(ReduceItem));
-    if (direct)
-      memcpy(dst + (off_children - off_tables), direct_children.data(),
+    if (!kvals.empty())
+      memcpy(dst + (off_kvals - off_tables), kvals.data(), kvals.size() * 4);
+    if (!direct_children.empty())
+      memcpy(dst + (off_dchildren - off_tables), direct_children.data(),
              direct_children.size() * sizeof(Child));
+    if (!direct_queries.empty())
+      memcpy(dst + (off_dqueries - off_tables), direct_queries.data(),
+             direct_queries.size() * sizeof(Query));
   }
 };
 
@@ -91,9 +99,9 @@ inline void plan_single(Plan& p, int kind, int dtype, uint64_t n, uint32_t nq,
     c.need_range = 1;
     p.direct_children.push_back(c);
     p.off_tables = off;
-    p.off_children = take(sizeof(Child));
+    p.off_dchildren = take(sizeof(Child));
     p.tables_end = off;
-    p.off_q = take((size_t)nq * sizeof(Query));
+    p.off_dqueries = take((size_t)nq * sizeof(Query));  // written by a query kernel
     p.total = off;
     return;
   }
@@ -176,6 +184,203 @@ inline void plan_single(Plan& p, int kind, int dtype, uint64_t n, uint32_t nq,
   p.off_children = take((size_t)nq * sizeof(Child));
   p.off_cq = take((size_t)nq * sizeof(Query));
   p.off_red = take((size_t)n * eb);
+  p.total = off;
+}
+
+// ------------------------------------------------------------------ batches
+// Per (dtype, floor(log2 length)) band: the bucket budget, the chunk size of
+// the streaming passes and whether a segment of the band goes straight to
+// the finisher (whole segment in LDS, no bucketing).  Defaults below; the
+// autotuner overwrites entries through ggks_tuning_set.
+struct BandTuning {
+  uint8_t buckets_log2;
+  uint8_t chunk_log2;  // elements per chunk
+  uint8_t direct;
+  uint8_t pad;
+};
+constexpr int kBands = 33;  // floor(log2 n) = 0 .. 32
+
+inline BandTuning band_default(int dtype, int band) {
+  const uint32_t cap = kFinishLdsBytes / (uint32_t)dtype_bytes(dtype);
+  BandTuning t = {};
+  if ((1ull << band) * 2 - 1 <= cap) {  // every length of the band fits in LDS
+    t.direct = 1;
+    return t;
+  }
+  const int bl = band <= 15 ? 8 : band <= 17 ? 10 : band <= 19 ? 11 : band <= 21 ? 12
+               : band <= 23 ? 13 : 15;
+  const int cl = band <= 15 ? 14 : band <= 17 ? 15 : band <= 19 ? 16 : band <= 21 ? 17
+               : band <= 23 ? 18 : 20;
+  t.buckets_log2 = (uint8_t)bl;
+  t.chunk_log2 = (uint8_t)cl;
+  return t;
+}
+
+struct TuningTable {
+  BandTuning t[4][kBands];
+  TuningTable() {
+    for (int d = 0; d < 4; ++d)
+      for (int b = 0; b < kBands; ++b) t[d][b] = band_default(d, b);
+  }
+};
+
+inline TuningTable& tuning_table() {
+  static TuningTable tab;
+  return tab;
+}
+
+inline int band_of(uint64_t n) {
+  int b = 0;
+  while (b < 32 && (n >> (b + 1))) ++b;
+  return b;
+}
+
+// One segment of a batch as the planner sees it (validated by the caller).
+struct BatchSeg {
+  uint64_t offset, length;
+  const uint32_t* k;  // host k values (multiselect), null for top-K
+  uint32_t num_k;     // k count, or K for a top-K
+  uint32_t kind;      // GGKS_KIND_*
+  uint64_t out_offset;
+};
+
+// Plan a batch: segments whose band is marked direct (and whose length fits
+// in the finisher's LDS) become direct children with explicit queries; the
+// rest become engine segments with their own chunk rows, LUT, bitmaps and a
+// slice of the reduced buffer.
+inline void plan_batch(Plan& p, int dtype, const std::vector<BatchSeg>& in, const ggks_opts& o) {
+  const int eb = dtype_bytes(dtype);
+  const uint64_t cap = kFinishLdsBytes / (uint32_t)eb;
+  const TuningTable& tab = tuning_table();
+  p.seed = o.seed ? o.seed : 0x6A09E667F3BCC909ull;
+  uint64_t red = 0, rowc = 0;
+  uint32_t qoff = 0, sam = 0, lutc = 0, totc = 0, bitc = 0, candc = 0, dq = 0;
+  for (size_t i = 0; i < in.size(); ++i) {
+    const BatchSeg& g = in[i];
+    if (g.num_k == 0) continue;
+    const uint64_t n = g.length;
+    const BandTuning bt = tab.t[dtype][band_of(n)];
+    const bool topk = g.kind == GGKS_KIND_TOPK;
+    if (n <= cap && (bt.direct || n <= 64)) {
+      Child c = {};
+      c.start = g.offset;
+      c.len = n;
+      c.q_off = dq;
+      c.q_cnt = g.num_k;
+      c.src = 0;
+      c.need_range = 1;
+      c.seg = (uint32_t)i;
+      p.direct_children.push_back(c);
+      for (uint32_t j = 0; j < g.num_k; ++j) {
+        Query q;
+        q.rank = topk ? (uint32_t)(n - j) : (uint32_t)(n - g.k[j] + 1);
+        q.out = (uint32_t)(g.out_offset + j);
+        p.direct_queries.push_back(q);
+      }
+      dq += g.num_k;
+      continue;
+    }
+    uint32_t mode = MODE_SAMPLE, rb = 0, bl, pb;
+    if (g.kind == GGKS_KIND_RADIX) {
+      mode = MODE_RADIX;
+      rb = o.radix_bits ? (uint32_t)o.radix_bits : 14u;
+      rb = std::min<uint32_t>(14, std::max<uint32_t>(4, rb));
+      bl = rb + 1;
+      pb = std::min<uint32_t>(kMaxPrefixBits, rb);
+    } else {
+      bl = o.buckets_log2 ? (uint32_t)o.buckets_log2 : (bt.buckets_log2 ? bt.buckets_log2 : 12u);
+      bl = std::min<uint32_t>(15, std::max<uint32_t>(8, bl));
+      pb = std::min<uint32_t>(kMaxPrefixBits, bl - 2);
+    }
+    const uint32_t bmax = 1u << bl;
+    uint64_t csz = 1ull << (bt.chunk_log2 ? bt.chunk_log2 : 16);
+    uint64_t G = (n + csz - 1) / csz;
+    G = std::max<uint64_t>(1, std::min<uint64_t>(G, kMaxChunks));
+    csz = (n + G - 1) / G;
+    csz = (csz + 63) / 64 * 64;
+    G = (n + csz - 1) / csz;
+    uint32_t ns = 0;
+    if (mode == MODE_SAMPLE) {
+      uint64_t want = (n / 256 + 1023) / 1024 * 1024;
+      ns = (uint32_t)std::min<uint64_t>(65536, std::max<uint64_t>(4096, want));
+    }
+    SegDesc s = {};
+    s.data_off = g.offset;
+    s.len = n;
+    s.row_base = rowc;
+    s.red_base = red;
+    s.bmax = bmax;
+    s.pbits = pb;
+    s.nchunks = (uint32_t)G;
+    s.sample_off = sam;
+    s.nsample = ns;
+    s.lut_off = lutc;
+    s.tot_off = totc;
+    s.bits_off = bitc;
+    s.cand_off = candc;
+    s.q_off = qoff;
+    s.q_cnt = g.num_k;
+    s.mode = mode;
+    s.radix_bits = rb;
+    s.qsrc = topk ? QSRC_TOPK : QSRC_K;
+    s.out_off = (uint32_t)g.out_offset;
+    const uint32_t si = (uint32_t)p.segs.size();
+    p.segs.push_back(s);
+    for (uint64_t c = 0; c < G; ++c) {
+      Chunk ch = {};
+      ch.seg = si;
+      ch.idx = (uint32_t)c;
+      ch.start = c * csz;
+      ch.len = std::min<uint64_t>(csz, n - c * csz);
+      p.chunks.push_back(ch);
+    }
+    for (uint32_t gi = 0; gi < (bmax + kReduceBins - 1) / kReduceBins; ++gi)
+      p.items.push_back({si, gi});
+    p.kvals.resize(qoff + g.num_k, 0u);
+    if (!topk) memcpy(p.kvals.data() + qoff, g.k, (size_t)g.num_k * 4);
+    p.any_sample |= mode == MODE_SAMPLE;
+    p.max_nsample = std::max(p.max_nsample, ns);
+    p.bmax_max = std::max(p.bmax_max, bmax);
+    p.pbits_max = std::max(p.pbits_max, pb);
+    red += n;
+    rowc += G * bmax;
+    qoff += g.num_k;
+    sam += ns;
+    lutc += 1u << pb;
+    totc += bmax;
+    bitc += (bmax + 31) / 32;
+    candc += (1u << pb) << (kCellBits - 5);
+  }
+  p.nq = qoff;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    size_t at = off;
+    off = align_up(off + bytes);
+    return at;
+  };
+  const size_t nseg = p.segs.size();
+  p.off_tables = off;
+  p.off_segs = take(nseg * sizeof(SegDesc));
+  p.off_chunks = take(p.chunks.size() * sizeof(Chunk));
+  p.off_items = take(p.items.size() * sizeof(ReduceItem));
+  p.off_kvals = take(p.kvals.size() * 4);
+  p.off_dchildren = take(p.direct_children.size() * sizeof(Child));
+  p.off_dqueries = take(p.direct_queries.size() * sizeof(Query));
+  p.tables_end = off;
+  p.off_states = take(nseg * sizeof(SegState));
+  p.off_samples = take((size_t)std::max<uint32_t>(sam, 1) * eb);
+  p.off_lut = take((size_t)lutc * 4);
+  p.off_rows = take((size_t)rowc * 4);
+  p.off_tot = take((size_t)totc * 4);
+  p.off_bits = take((size_t)bitc * 4);
+  p.off_wbase = take((size_t)bitc * 4);
+  p.off_cand = take((size_t)candc * 4);
+  p.off_q = off;  // engine segments read k lists or top-K ranks, no Query records
+  p.off_qb = take((size_t)qoff * 4);
+  p.off_qr = take((size_t)qoff * 4);
+  p.off_children = take((size_t)qoff * sizeof(Child));
+  p.off_cq = take((size_t)qoff * sizeof(Query));
+  p.off_red = take((size_t)red * eb);
   p.total = off;
 }
 

@@ -17,6 +17,7 @@
 #include "ggks.h"
 #include "ggks_kernels.hpp"
 #include "ggks_plan.hpp"
+#include "ggks_gen.hpp"
 
 namespace ggks {
 
@@ -141,96 +142,121 @@ static void set_lds_limit(K kernel, size_t bytes) {
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
+// Where one launch sequence runs: its arena (plan tables + scratch), stream,
+// status word and, when stage timing is armed, the context recording it.
+struct Exec {
+  char* arena;
+  hipStream_t stream;
+  uint32_t* status;
+  ggks_ctx* prof;  // null: no stage events (also during graph capture)
+};
+
+static void stage(const Exec& x, int st) {
+  if (x.prof) mark(x.prof, st);
+}
+
+// Dynamic-LDS limits of the level kernels: the largest size any plan can
+// request, set once per instantiation (the limit is a per-function attribute
+// shared by every plan and graph; occupancy follows the size each launch
+// actually asks for).
 template <typename KT, int C>
-static int launch_level(ggks_ctx* c, const Plan& p, const KT* d_src, KT* d_out,
-                        const uint32_t* d_k) {
-  char* A = c->arena;
-  const SegDesc* segs = reinterpret_cast<const SegDesc*>(A + p.off_segs);
-  SegState* states = reinterpret_cast<SegState*>(A + p.off_states);
-  const Chunk* chunks = reinterpret_cast<const Chunk*>(A + p.off_chunks);
-  const ReduceItem* items = reinterpret_cast<const ReduceItem*>(A + p.off_items);
-  KT* samples = reinterpret_cast<KT*>(A + p.off_samples);
-  uint32_t* lut = reinterpret_cast<uint32_t*>(A + p.off_lut);
-  uint32_t* rows = reinterpret_cast<uint32_t*>(A + p.off_rows);
-  uint32_t* tot = reinterpret_cast<uint32_t*>(A + p.off_tot);
-  uint32_t* bits = reinterpret_cast<uint32_t*>(A + p.off_bits);
-  uint32_t* wbase = reinterpret_cast<uint32_t*>(A + p.off_wbase);
-  uint32_t* cand = reinterpret_cast<uint32_t*>(A + p.off_cand);
-  const Query* q = reinterpret_cast<const Query*>(A + p.off_q);
-  uint32_t* qb = reinterpret_cast<uint32_t*>(A + p.off_qb);
-  uint32_t* qr = reinterpret_cast<uint32_t*>(A + p.off_qr);
-  Child* kids = reinterpret_cast<Child*>(A + p.off_children);
-  Query* cq = reinterpret_cast<Query*>(A + p.off_cq);
-  KT* red = reinterpret_cast<KT*>(A + p.off_red);
-  hipStream_t s = c->stream;
+static void level_attrs(const Plan&) {
+  static const bool once = [] {
+    set_lds_limit(k_count<KT, C>, ((size_t)kMaxBuckets + (1u << kMaxPrefixBits)) * 4);
+    set_lds_limit(k_findk<KT, C>, ((size_t)kMaxBuckets + (1u << kMaxPrefixBits) +
+                                   3 * ((kMaxBuckets + 31) / 32) + 32) * 4);
+    set_lds_limit(k_compact<KT, C>, compact_lds_bytes<KT>(kMaxPrefixBits, kMaxBuckets));
+    set_lds_limit(k_finish<KT, C>, kFinishLdsTotal);
+    return true;
+  }();
+  (void)once;
+}
+
+// The level pipeline over every engine segment of the plan, then the
+// finisher over the plan's direct children (whole small segments).
+template <typename KT, int C>
+static int enqueue_level(const Exec& x, const Plan& p, const KT* d_src, KT* d_out,
+                         const uint32_t* d_k) {
+  char* A = x.arena;
+  hipStream_t s = x.stream;
   const uint32_t nseg = (uint32_t)p.segs.size();
+  if (nseg > 0) {
+    const SegDesc* segs = reinterpret_cast<const SegDesc*>(A + p.off_segs);
+    SegState* states = reinterpret_cast<SegState*>(A + p.off_states);
+    const Chunk* chunks = reinterpret_cast<const Chunk*>(A + p.off_chunks);
+    const ReduceItem* items = reinterpret_cast<const ReduceItem*>(A + p.off_items);
+    KT* samples = reinterpret_cast<KT*>(A + p.off_samples);
+    uint32_t* lut = reinterpret_cast<uint32_t*>(A + p.off_lut);
+    uint32_t* rows = reinterpret_cast<uint32_t*>(A + p.off_rows);
+    uint32_t* tot = reinterpret_cast<uint32_t*>(A + p.off_tot);
+    uint32_t* bits = reinterpret_cast<uint32_t*>(A + p.off_bits);
+    uint32_t* wbase = reinterpret_cast<uint32_t*>(A + p.off_wbase);
+    uint32_t* cand = reinterpret_cast<uint32_t*>(A + p.off_cand);
+    const Query* q = reinterpret_cast<const Query*>(A + p.off_q);
+    uint32_t* qb = reinterpret_cast<uint32_t*>(A + p.off_qb);
+    uint32_t* qr = reinterpret_cast<uint32_t*>(A + p.off_qr);
+    Child* kids = reinterpret_cast<Child*>(A + p.off_children);
+    Query* cq = reinterpret_cast<Query*>(A + p.off_cq);
+    KT* red = reinterpret_cast<KT*>(A + p.off_red);
+    const uint32_t nchunks = (uint32_t)p.chunks.size();
 
-  mark(c, 1);
-  if (p.any_sample) {
-    dim3 g(nseg, (p.max_nsample + kThreads - 1) / kThreads);
-    hipLaunchKernelGGL((k_sample<KT, C>), g, dim3(kThreads), 0, s, d_src, segs, samples, p.seed);
+    stage(x, 1);
+    if (p.any_sample) {
+      dim3 g(nseg, (p.max_nsample + kThreads - 1) / kThreads);
+      hipLaunchKernelGGL((k_sample<KT, C>), g, dim3(kThreads), 0, s, d_src, segs, samples, p.seed);
+    }
+    stage(x, 2);
+    hipLaunchKernelGGL((k_lut<KT>), dim3(nseg), dim3(kThreads), 0, s, segs, samples, states, lut);
+    stage(x, 3);
+    hipLaunchKernelGGL((k_count<KT, C>), dim3(nchunks), dim3(kThreads),
+                       ((size_t)p.bmax_max + (1u << p.pbits_max)) * 4, s, d_src, segs, states,
+                       chunks, lut, rows);
+    stage(x, 4);
+    hipLaunchKernelGGL(k_reduce, dim3((uint32_t)p.items.size()), dim3(kThreads), 0, s, segs,
+                       states, items, rows, tot);
+    stage(x, 5);
+    hipLaunchKernelGGL((k_findk<KT, C>), dim3(nseg), dim3(kThreads),
+                       ((size_t)p.bmax_max + (1u << p.pbits_max) + 3 * ((p.bmax_max + 31) / 32) +
+                        32) * 4,
+                       s, segs, states, lut, tot, q, d_k, x.status, qb, qr, kids, cq, bits, wbase,
+                       cand, d_out);
+    stage(x, 6);
+    hipLaunchKernelGGL((k_compact<KT, C>), dim3(nchunks), dim3(kThreads),
+                       compact_lds_bytes<KT>(p.pbits_max, p.bmax_max), s, d_src, segs, states,
+                       chunks, lut, rows, bits, wbase, cand, kids, red);
+    stage(x, 7);
+    const uint32_t fgrid = std::min<uint32_t>(p.nq, 2048);
+    hipLaunchKernelGGL((k_finish<KT, C>), dim3(fgrid), dim3(kThreads), kFinishLdsTotal, s, d_src,
+                       red, kids, p.nq, cq, d_out);
+  } else {
+    for (int st = 1; st < GGKS_NSTAGES; ++st) stage(x, st);
   }
-  mark(c, 2);
-  hipLaunchKernelGGL((k_lut<KT>), dim3(nseg), dim3(kThreads), 0, s, segs, samples, states, lut);
-  mark(c, 3);
-
-  const size_t lds_count = ((size_t)p.bmax_max + (1u << p.pbits_max)) * 4;
-  set_lds_limit(k_count<KT, C>, lds_count);
-  hipLaunchKernelGGL((k_count<KT, C>), dim3((uint32_t)p.chunks.size()), dim3(kThreads),
-                     lds_count, s, d_src, segs, states, chunks, lut, rows);
-  mark(c, 4);
-  hipLaunchKernelGGL(k_reduce, dim3((uint32_t)p.items.size()), dim3(kThreads), 0, s, segs,
-                     states, items, rows, tot);
-  mark(c, 5);
-  const size_t lds_findk =
-      ((size_t)p.bmax_max + (1u << p.pbits_max) + 3 * ((p.bmax_max + 31) / 32) + 32) * 4;
-  set_lds_limit(k_findk<KT, C>, lds_findk);
-  hipLaunchKernelGGL((k_findk<KT, C>), dim3(nseg), dim3(kThreads), lds_findk, s, segs, states,
-                     lut, tot, q, d_k, c->status, qb, qr, kids, cq, bits, wbase, cand, d_out);
-  const size_t lds_compact = compact_lds_bytes<KT>(p.pbits_max, p.bmax_max);
-  set_lds_limit(k_compact<KT, C>, lds_compact);
-  mark(c, 6);
-  hipLaunchKernelGGL((k_compact<KT, C>), dim3((uint32_t)p.chunks.size()), dim3(kThreads),
-                     lds_compact, s, d_src, segs, states, chunks, lut, rows, bits, wbase, cand,
-                     kids, red);
-  set_lds_limit(k_finish<KT, C>, kFinishLdsTotal);
-  mark(c, 7);
-  const uint32_t fgrid = std::min<uint32_t>(p.nq, 2048);
-  hipLaunchKernelGGL((k_finish<KT, C>), dim3(fgrid), dim3(kThreads), kFinishLdsTotal, s, d_src,
-                     red, kids, p.nq, cq, d_out);
-  HIP_TRY(hipGetLastError());
-  return GGKS_OK;
-}
-
-// Small vector: one child on the input, finished in LDS.
-template <typename KT, int C>
-static int launch_direct(ggks_ctx* c, const Plan& p, const KT* d_src, KT* d_out) {
-  char* A = c->arena;
-  const Child* kids = reinterpret_cast<const Child*>(A + p.off_children);
-  const Query* q = reinterpret_cast<const Query*>(A + p.off_q);
-  set_lds_limit(k_finish<KT, C>, kFinishLdsTotal);
-  for (int st = 1; st < GGKS_NSTAGES; ++st) mark(c, st);
-  hipLaunchKernelGGL((k_finish<KT, C>), dim3(1), dim3(kThreads), kFinishLdsTotal, c->stream,
-                     d_src, (const KT*)nullptr, kids, 1u, q, d_out);
+  const uint32_t ndirect = (uint32_t)p.direct_children.size();
+  if (ndirect > 0) {
+    const Child* dk = reinterpret_cast<const Child*>(A + p.off_dchildren);
+    const Query* dq = reinterpret_cast<const Query*>(A + p.off_dqueries);
+    hipLaunchKernelGGL((k_finish<KT, C>), dim3(std::min<uint32_t>(ndirect, 4096)),
+                       dim3(kThreads), kFinishLdsTotal, s, d_src, (const KT*)nullptr, dk, ndirect,
+                       dq, d_out);
+  }
   HIP_TRY(hipGetLastError());
   return GGKS_OK;
 }
 
 template <typename KT, int C>
-static int launch_typed(ggks_ctx* c, const Plan& p, const void* d_src, void* d_out,
+static int launch_typed(const Exec& x, const Plan& p, const void* d_src, void* d_out,
                         const uint32_t* d_k) {
-  if (p.direct)
-    return launch_direct<KT, C>(c, p, (const KT*)d_src, (KT*)d_out);
-  return launch_level<KT, C>(c, p, (const KT*)d_src, (KT*)d_out, d_k);
+  level_attrs<KT, C>(p);
+  return enqueue_level<KT, C>(x, p, (const KT*)d_src, (KT*)d_out, d_k);
 }
 
-static int launch(ggks_ctx* c, int dtype, const Plan& p, const void* d_src, void* d_out,
+static int launch(const Exec& x, int dtype, const Plan& p, const void* d_src, void* d_out,
                   const uint32_t* d_k) {
   switch (dtype) {
-    case GGKS_F32: return launch_typed<uint32_t, CONV_F>(c, p, d_src, d_out, d_k);
-    case GGKS_F64: return launch_typed<uint64_t, CONV_F>(c, p, d_src, d_out, d_k);
-    case GGKS_U32: return launch_typed<uint32_t, CONV_U>(c, p, d_src, d_out, d_k);
-    case GGKS_I32: return launch_typed<uint32_t, CONV_I>(c, p, d_src, d_out, d_k);
+    case GGKS_F32: return launch_typed<uint32_t, CONV_F>(x, p, d_src, d_out, d_k);
+    case GGKS_F64: return launch_typed<uint64_t, CONV_F>(x, p, d_src, d_out, d_k);
+    case GGKS_U32: return launch_typed<uint32_t, CONV_U>(x, p, d_src, d_out, d_k);
+    case GGKS_I32: return launch_typed<uint32_t, CONV_I>(x, p, d_src, d_out, d_k);
   }
   return fail(GGKS_EINVAL, "bad dtype %d", dtype);
 }
@@ -330,7 +356,7 @@ static int select_one(ggks_ctx* c, int kind, int dtype, const void* d_vec, int64
     d_k = reinterpret_cast<const uint32_t*>(c->arena + off_dk);
   }
   if (p.direct) {  // the direct path takes explicit Query records
-    Query* d_q = reinterpret_cast<Query*>(c->arena + p.off_q);
+    Query* d_q = reinterpret_cast<Query*>(c->arena + p.off_dqueries);
     if (ksrc == KSource::TOPK)
       hipLaunchKernelGGL(k_topk_queries, dim3(qgrid), dim3(256), 0, c->stream, nk, (uint64_t)n,
                          d_q);
@@ -339,7 +365,8 @@ static int select_one(ggks_ctx* c, int kind, int dtype, const void* d_vec, int64
                          d_q, c->status);
   }
   void* d_out = host_out ? (void*)(c->arena + off_dout) : out;
-  rc = launch(c, dtype, p, d_vec, d_out, d_k);
+  const Exec x = {c->arena, c->stream, c->status, c->prof_cur >= 0 ? c : nullptr};
+  rc = launch(x, dtype, p, d_vec, d_out, d_k);
   mark(c, GGKS_NSTAGES);
   c->prof_cur = -1;
   if (rc) return rc;
@@ -353,6 +380,8 @@ static int select_one(ggks_ctx* c, int kind, int dtype, const void* d_vec, int64
   }
   return GGKS_OK;
 }
+
+#include "ggks_batch.hpp"
 
 }  // namespace ggks
 
@@ -524,6 +553,18 @@ int ggks_d2h(void* dst, const void* src, size_t bytes) {
   return GGKS_OK;
 }
 int ggks_d2d(void* dst, const void* src, size_t bytes) {
+  // 16-byte aligned whole-vector copies run the library's own streaming copy
+  // kernel (one read and one write of every byte: a known-size reference for
+  // memory-traffic counters); anything else goes through the runtime.
+  if (((uintptr_t)dst | (uintptr_t)src | bytes) % 16 == 0 && bytes > 0) {
+    const uint64_t nv = bytes / 16;
+    const uint32_t grid = (uint32_t)std::min<uint64_t>((nv + kThreads - 1) / kThreads, 4096);
+    hipLaunchKernelGGL(k_copy16, dim3(grid), dim3(kThreads), 0, nullptr,
+                       reinterpret_cast<const uint4*>(src), reinterpret_cast<uint4*>(dst), nv);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipDeviceSynchronize());
+    return GGKS_OK;
+  }
   HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
   return GGKS_OK;
 }
@@ -541,6 +582,98 @@ int ggks_set_device(int device) {
 }
 int ggks_dev_fill(void* ptr, int value, size_t bytes) {
   HIP_TRY(hipMemset(ptr, value, bytes));
+  return GGKS_OK;
+}
+
+
+// ---------------------------------------------------------------- batches
+int ggks_batch_create(ggks_batch** batch, int device, int dtype, const ggks_segment* segs,
+                      int64_t num_segments, const uint32_t* h_k, int64_t total_k,
+                      const ggks_opts* opt, void* d_scratch, size_t scratch_bytes,
+                      void* stream) {
+  return batch_create(batch, device, dtype, segs, num_segments, h_k, total_k, opt, d_scratch,
+                      scratch_bytes, stream);
+}
+
+size_t ggks_batch_scratch_bytes(int dtype, const ggks_segment* segs, int64_t num_segments,
+                                const uint32_t* h_k, int64_t total_k, const ggks_opts* opt) {
+  std::vector<BatchSeg> in;
+  uint64_t elements = 0;
+  if (batch_segments(dtype, segs, num_segments, h_k, total_k, in, elements)) return 0;
+  ggks_opts o = {};
+  if (opt) o = *opt;
+  Plan p;
+  plan_batch(p, dtype, in, o);
+  return batch_arena_bytes(p);
+}
+
+int ggks_batch_run(ggks_batch* b, const void* d_vec, void* d_out, int flags) {
+  return batch_run(b, d_vec, d_out, flags);
+}
+
+int ggks_batch_sync(ggks_batch* b) { return batch_sync(b); }
+
+void* ggks_batch_stream(ggks_batch* b) { return b ? (void*)b->stream : nullptr; }
+
+int ggks_batch_info(ggks_batch* b, ggks_batch_stats* info) {
+  if (!b || !info) return fail(GGKS_EINVAL, "null argument");
+  info->elements = b->elements;
+  info->queries = b->plan.nq + b->plan.direct_queries.size();
+  info->scratch_bytes = b->arena_bytes;
+  info->engine_segments = b->engine;
+  info->direct_segments = b->direct;
+  info->chunks = (uint32_t)b->plan.chunks.size();
+  info->graph = b->exec ? 1 : 0;
+  return GGKS_OK;
+}
+
+int ggks_batch_destroy(ggks_batch* b) { return batch_destroy(b); }
+
+int ggks_tuning_set(int dtype, int log2_len, int buckets_log2, int chunk_log2, int direct) {
+  if (dtype < 0 || dtype > 3 || log2_len < 0 || log2_len >= kBands)
+    return fail(GGKS_EINVAL, "bad tuning band (dtype %d, log2 length %d)", dtype, log2_len);
+  if (buckets_log2 < 8 || buckets_log2 > 15 || chunk_log2 < 10 || chunk_log2 > 26)
+    return fail(GGKS_EINVAL, "bad tuning values (buckets 2^%d, chunk 2^%d)", buckets_log2,
+                chunk_log2);
+  BandTuning t = {};
+  t.buckets_log2 = (uint8_t)buckets_log2;
+  t.chunk_log2 = (uint8_t)chunk_log2;
+  t.direct = direct ? 1 : 0;
+  tuning_table().t[dtype][log2_len] = t;
+  return GGKS_OK;
+}
+
+int ggks_tuning_get(int dtype, int log2_len, int* buckets_log2, int* chunk_log2, int* direct) {
+  if (dtype < 0 || dtype > 3 || log2_len < 0 || log2_len >= kBands)
+    return fail(GGKS_EINVAL, "bad tuning band (dtype %d, log2 length %d)", dtype, log2_len);
+  const BandTuning t = tuning_table().t[dtype][log2_len];
+  if (buckets_log2) *buckets_log2 = t.buckets_log2;
+  if (chunk_log2) *chunk_log2 = t.chunk_log2;
+  if (direct) *direct = t.direct;
+  return GGKS_OK;
+}
+
+int ggks_tuning_reset(void) {
+  tuning_table() = TuningTable();
+  return GGKS_OK;
+}
+
+// ---------------------------------------------------------- device inputs
+int ggks_generate(void* d_ptr, int dtype, int family, uint64_t seed, uint64_t start, uint64_t n,
+                  int64_t lo, int64_t hi) {
+  if (!d_ptr && n) return fail(GGKS_EINVAL, "null pointer");
+  if (dtype < 0 || dtype > 3 || family < 0 || family > 1)
+    return fail(GGKS_EINVAL, "bad dtype %d or family %d", dtype, family);
+  if ((dtype == GGKS_U32 || dtype == GGKS_I32) &&
+      (hi < lo || (dtype == GGKS_U32 && (lo < 0 || hi > 0xFFFFFFFFll)) ||
+       (dtype == GGKS_I32 && (lo < INT32_MIN || hi > INT32_MAX))))
+    return fail(GGKS_EINVAL, "bad integer range [%lld, %lld]", (long long)lo, (long long)hi);
+  if (n == 0) return GGKS_OK;
+  const uint32_t grid = (uint32_t)std::min<uint64_t>((n + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_generate, dim3(grid), dim3(256), 0, nullptr, d_ptr, dtype, family, seed,
+                     start, n, lo, hi);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipDeviceSynchronize());
   return GGKS_OK;
 }
 

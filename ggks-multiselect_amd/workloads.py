@@ -129,3 +129,40 @@ def k_cluster(seed: int, n: int, count: int = 256, width: int = 1000) -> np.ndar
     keys = splitmix64(seed, 0, width)
     pick = np.sort(np.argsort(keys, kind="stable")[:count]) + lo
     return np.concatenate([pick.astype(np.uint32), np.array([1, n], np.uint32)])
+
+
+# ---------------------------------------------------------------- batches
+def batch_spec(master_seed: int, nseg: int, log2_lo: float = 10.0, log2_hi: float = 22.0,
+               max_k: int = 64, max_topk: int = 4096, topk_frac: float = 0.5) -> dict:
+    """An uneven batch: segment lengths floor(2^U), U ~ U[log2_lo, log2_hi];
+    values U[0,1) or N(0,1) by a seeded coin (per-segment seed from the
+    master stream); each segment asks either 1..max_k uniform k values in
+    [1, len] or a top-K with K = floor(2^V), V ~ U[0, log2(min(max_topk, len))].
+    Returns lengths, kinds (ggks kind codes), num_k, the concatenated k list,
+    per-segment value seeds and families."""
+    z = splitmix64(master_seed, 0, 6 * nseg).reshape(nseg, 6)
+    unit = lambda w: (w >> np.uint64(11)).astype(np.float64) * (2.0 ** -53)  # noqa: E731
+    lengths = np.floor(2.0 ** (log2_lo + unit(z[:, 0]) * (log2_hi - log2_lo))).astype(np.uint64)
+    lengths = np.maximum(lengths, 1)
+    normal = (z[:, 1] & np.uint64(1)).astype(bool)
+    topk = unit(z[:, 2]) < topk_frac
+    nk = (1 + z[:, 3] % np.uint64(max_k)).astype(np.uint32)
+    kmax = np.minimum(lengths, max_topk).astype(np.float64)
+    K = np.floor(2.0 ** (unit(z[:, 4]) * np.log2(kmax))).astype(np.uint32)
+    K = np.clip(K, 1, kmax.astype(np.uint32))
+    seeds = z[:, 5]
+    num_k = np.where(topk, K, nk).astype(np.uint32)
+    klists = []
+    for i in np.nonzero(~topk)[0]:
+        klists.append((splitmix64(int(seeds[i]), 0, int(nk[i])) % lengths[i] + np.uint64(1))
+                      .astype(np.uint32))
+    k = np.concatenate(klists) if klists else np.zeros(0, np.uint32)
+    return {"lengths": lengths, "kinds": np.where(topk, 2, 0).astype(np.uint32),
+            "num_k": num_k, "k": k, "seeds": seeds, "normal": normal}
+
+
+def batch_values_f32(spec: dict) -> np.ndarray:
+    """The batch vector of a batch_spec on the host (segments back to back)."""
+    parts = [(normal_f32 if nm else uniform_f32)(int(sd), int(n))
+             for n, sd, nm in zip(spec["lengths"], spec["seeds"], spec["normal"])]
+    return np.concatenate(parts) if parts else np.zeros(0, np.float32)

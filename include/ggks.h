@@ -157,6 +157,89 @@ int ggks_set_device(int device);
 /* Fill a device buffer with a byte pattern (L2/MALL flush helper). */
 int ggks_dev_fill(void *ptr, int value, size_t bytes);
 
+/* ---- batched selection over a segment table --------------------------
+ * No reference counterpart (the reference selects one vector per call).
+ * Many independent vectors of uneven length, each with its own k list or
+ * top-K, selected together: one launch per engine kernel for all segments
+ * (plus one finisher launch for every small segment), optionally replayed
+ * from a captured hipGraph.  Every output equals what the single-vector
+ * entry point returns for that segment alone.
+ *
+ * A batch vector holds all segments (one dtype); segment i is
+ * vec[offset, offset + length).  Multiselect segments (kind BUCKET or RADIX)
+ * take k values h_k[k_offset, k_offset + num_k) (1 <= k <= length) and write
+ * out[out_offset + j] = the k_j-th largest; top-K segments (kind TOPK,
+ * num_k = K <= length) write the K largest, descending, from out_offset.
+ * Segments with num_k = 0 are skipped.  Output ranges must not overlap. */
+typedef struct ggks_segment {
+  uint64_t offset;     /* first element of the segment in the batch vector */
+  uint64_t length;     /* elements, 1 <= length < 2^32 */
+  uint64_t k_offset;   /* multiselect: first of its k values in h_k */
+  uint64_t out_offset; /* first output slot; out_offset + num_k <= 2^32 */
+  uint32_t num_k;      /* multiselect: number of k values; top-K: K */
+  uint32_t kind;       /* GGKS_KIND_BUCKET, GGKS_KIND_RADIX or GGKS_KIND_TOPK */
+} ggks_segment;
+
+typedef struct ggks_batch ggks_batch;
+
+typedef struct ggks_batch_stats {
+  uint64_t elements;        /* sum of the lengths of segments with queries */
+  uint64_t queries;         /* outputs written per run */
+  uint64_t scratch_bytes;   /* device bytes of tables + scratch */
+  uint32_t engine_segments; /* segments bucketed by the level kernels */
+  uint32_t direct_segments; /* small segments finished whole in LDS */
+  uint32_t chunks;          /* work-groups of each streaming pass */
+  uint32_t graph;           /* 1 once a hipGraph has been captured */
+} ggks_batch_stats;
+
+#define GGKS_BATCH_GRAPH 1 /* ggks_batch_run flag: capture once, then replay */
+
+/* Plan a batch and upload its tables (the k lists are copied; h_k may be
+ * freed afterwards).  d_scratch: device memory of at least
+ * ggks_batch_scratch_bytes() bytes, or NULL to let the batch allocate its
+ * own.  stream: hipStream_t to run on, or NULL for a stream of its own (a
+ * graph cannot be captured on the null stream).  The band tuning table
+ * (ggks_tuning_set) in force at creation decides each segment's bucket
+ * budget, chunk size and direct / engine route. */
+int ggks_batch_create(ggks_batch **batch, int device, int dtype, const ggks_segment *segs,
+                      int64_t num_segments, const uint32_t *h_k, int64_t total_k,
+                      const ggks_opts *opt, void *d_scratch, size_t scratch_bytes,
+                      void *stream);
+/* 0 on a bad table (ggks_last_error tells why). */
+size_t ggks_batch_scratch_bytes(int dtype, const ggks_segment *segs, int64_t num_segments,
+                                const uint32_t *h_k, int64_t total_k, const ggks_opts *opt);
+/* Enqueue one run on the batch's stream (asynchronous).  d_vec: the batch
+ * vector; d_out: device outputs.  flags: GGKS_BATCH_GRAPH replays a hipGraph
+ * captured on the first run with these two pointers (recaptured when they
+ * change).  Runs of one batch share its scratch: they are stream-ordered. */
+int ggks_batch_run(ggks_batch *batch, const void *d_vec, void *d_out, int flags);
+/* Wait for the batch's stream; GGKS_EINVAL if a kernel flagged bad input. */
+int ggks_batch_sync(ggks_batch *batch);
+void *ggks_batch_stream(ggks_batch *batch);
+int ggks_batch_info(ggks_batch *batch, ggks_batch_stats *info);
+int ggks_batch_destroy(ggks_batch *batch);
+
+/* Band tuning table used by batch planning, per dtype and floor(log2 length):
+ * bucket budget 2^buckets_log2 (8..15), chunk of 2^chunk_log2 elements
+ * (10..26) per work-group of the streaming passes, and direct = 1 to finish
+ * segments of the band whole in LDS when they fit.  The autotuner of the
+ * Python package measures and sets these; ggks_tuning_reset restores the
+ * built-in defaults.  Not synchronised with concurrent batch creation. */
+int ggks_tuning_set(int dtype, int log2_len, int buckets_log2, int chunk_log2, int direct);
+int ggks_tuning_get(int dtype, int log2_len, int *buckets_log2, int *chunk_log2, int *direct);
+int ggks_tuning_reset(void);
+
+/* ---- seeded synthetic inputs on the device ------------------------------
+ * Replaces the reference's cuRAND problem generators (generateProblems.cu:
+ * 25-428, clock-seeded there).  Element i is drawn from output start + i of
+ * the SplitMix64 stream of `seed`.  family 0 = uniform: f32 on the 2^-24
+ * grid of [0,1), f64 on the 2^-53 grid, u32/i32 uniform in [lo, hi]
+ * (bit-identical to the Python package's host generators); family 1 =
+ * normal(0,1) by Box-Muller in double (f32/f64; the device's libm may differ
+ * from the host's in the last bit).  Synchronous. */
+int ggks_generate(void *d_ptr, int dtype, int family, uint64_t seed, uint64_t start, uint64_t n,
+                  int64_t lo, int64_t hi);
+
 #ifdef __cplusplus
 }
 #endif

@@ -95,3 +95,36 @@ def test_workload_generators_are_pinned(pkg):
     assert kc[:-2].min() >= (1 << 26) - 500 and kc[:-2].max() <= (1 << 26) + 499
     assert np.array_equal(W.k_percentiles(101), np.arange(1, 102)[[round(p) for p in
                                                                    np.arange(101) * 1.0]])
+
+
+def test_batch_validation_without_device(pkg, L):
+    lib = L.lib()
+    table = pkg.segment_table([100, 50, 10], ["bucket", "topk", "radix"], [3, 50, 2])
+    k = np.array([1, 100, 7, 10, 1], np.uint32)
+    assert pkg.Batch.scratch_bytes(np.float32, table, k) > 100 * 4
+    import ctypes
+    for bad_k, bad_table in (
+            (np.array([1, 101, 7, 10, 1], np.uint32), table),       # k > length
+            (np.array([0, 100, 7, 10, 1], np.uint32), table),       # k = 0
+            (k[:4], table)):                                         # k list too short
+        h = ctypes.c_void_p()
+        rc = lib.ggks_batch_create(ctypes.byref(h), 0, L.F32, bad_table.ctypes.data, bad_table.size,
+                                   bad_k.ctypes.data, bad_k.size, None, None, 0, None)
+        assert rc == L.EINVAL and lib.ggks_last_error()
+    t2 = table.copy()
+    t2["num_k"][1] = 51  # K > length
+    t3 = table.copy()
+    t3["kind"][0] = 9
+    t4 = table.copy()
+    t4["length"][2] = 0
+    for t in (t2, t3, t4):
+        assert pkg.Batch.scratch_bytes(np.float32, t, k) == 0
+    with pytest.raises(pkg.GGKSError):
+        pkg.tuning_set(np.float32, 40, 12, 16, False)
+    with pytest.raises(pkg.GGKSError):
+        pkg.tuning_set(np.float32, 20, 16, 16, False)
+    pkg.tuning_set(np.float32, 20, 12, 16, False)
+    assert pkg.tuning_get(np.float32, 20) == (12, 16, False)
+    pkg.tuning_reset()
+    assert pkg.tuning_get(np.float32, 12)[2] is True   # 2^12..2^13-1 f32 fit in LDS: direct
+    assert pkg.tuning_get(np.float32, 14)[2] is False  # up to 2^15-1 does not

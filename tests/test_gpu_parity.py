@@ -172,3 +172,17 @@ def test_options_do_not_change_results(pkg, O):
         check(pkg, O, x, k, **opt)
     for rb in (4, 8, 11, 14):
         check(pkg, O, x, k, kind="radix", radix_bits=rb)
+
+
+def test_device_copy_plumbing(pkg):
+    from ggks_multiselect_amd import _lib as L
+    W = pkg.workloads
+    x = W.uint_range(21, (1 << 20) + 7, 0, 2**32 - 1)
+    d = pkg.DeviceArray.from_host(x)
+    for nbytes in (16, 4096, (1 << 20) * 4, 4 * x.size, 4 * x.size - 12):  # kernel and runtime paths
+        y = pkg.DeviceArray(x.size, np.uint32)
+        L.check(L.lib().ggks_dev_fill(y.ptr, 0, 4 * x.size))
+        L.check(L.lib().ggks_d2d(y.ptr, d.ptr, nbytes))
+        h = y.to_host()
+        m = nbytes // 4
+        assert np.array_equal(h[:m], x[:m]) and not np.any(h[m:])
