@@ -48,6 +48,12 @@ def parse():
     ap.add_argument("--buckets-log2", type=int, default=0, help="bucket budget (0 = tuned)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-looped leg")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--serve-seconds", type=float, default=20.0,
+                    help="length of each served trace (the workload definition uses 60)")
+    ap.add_argument("--serve-rates", default="2000,8000,32000")
+    ap.add_argument("--batch-segments", type=int, default=4096)
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the timed steps, copy the vector once with the library's plain "
                          "16-byte copy kernel (known-size reference for memory-traffic counters)")
@@ -102,6 +108,112 @@ def cpu_baseline(pkg, x: np.ndarray, numk: int, budget_s: float):
         "ms": round(t * 1e3, 2),
         "gpu_parity": parity,
     }
+
+
+def batched_leg(pkg, L, nseg: int):
+    """4096 uneven f32 vectors (lengths floor(2^U), U ~ U[10, 22]; uniform or
+    normal values; 1..64 k or a top-K each) selected by one batch (graph
+    replay) against looping the single-vector entry point over the same
+    segments; outputs must be identical, and 64 segments are checked against
+    a host sort."""
+    import ctypes
+    lib = L.lib()
+    pkg.autotune.apply()
+    spec = pkg.workloads.batch_spec(4096, nseg, 10, 22)
+    table = pkg.segment_table(spec["lengths"], spec["kinds"], spec["num_k"])
+    N = int(spec["lengths"].sum())
+    d = pkg.DeviceArray(N, np.float32)
+    for i, srow in enumerate(table):
+        pkg.generate(np.float32, "normal" if spec["normal"][i] else "uniform",
+                     int(spec["seeds"][i]), int(srow["length"]), out=d, offset=int(srow["offset"]))
+    b = pkg.Batch(np.float32, table, spec["k"])
+    out = pkg.DeviceArray(b.num_outputs, np.float32)
+
+    def timed(fn, reps):
+        L.check(lib.ggks_sync())
+        t = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        L.check(lib.ggks_sync())
+        return (time.perf_counter() - t) / reps
+
+    b.run(d, out, graph=True)
+    b.sync()
+    t_batch = timed(lambda: b.run(d, out, graph=True), 10)
+    got = out.to_host()
+    ctx = pkg.Context(0)
+    dk = pkg.DeviceArray.from_host(spec["k"] if spec["k"].size else np.ones(1, np.uint32))
+    lout = pkg.DeviceArray(b.num_outputs, np.float32)
+    o = pkg.api._opts()
+    calls = []
+    for srow in table:
+        vp, op = d.ptr + int(srow["offset"]) * 4, lout.ptr + int(srow["out_offset"]) * 4
+        if srow["kind"] == L.KIND_TOPK:
+            calls.append((lib.ggks_topk_async, (ctx.handle, L.F32, vp, int(srow["length"]),
+                                                int(srow["num_k"]), op, ctypes.byref(o))))
+        else:
+            calls.append((lib.ggks_bucket_multiselect_async,
+                          (ctx.handle, L.F32, vp, int(srow["length"]),
+                           dk.ptr + int(srow["k_offset"]) * 4, int(srow["num_k"]), op,
+                           ctypes.byref(o))))
+
+    def loop():
+        for f, args in calls:
+            L.check(f(*args))
+
+    loop()
+    t_loop = timed(loop, 2)
+    identical = bool(np.array_equal(lout.to_host().view(np.uint32), got.view(np.uint32)))
+    # spot check against a host sort
+    pick = np.linspace(0, nseg - 1, 64).astype(int)
+    host_ok = True
+    for i in pick:
+        srow = table[i]
+        seg = np.empty(int(srow["length"]), np.float32)
+        L.check(lib.ggks_d2h(seg.ctypes.data, d.ptr + int(srow["offset"]) * 4, seg.nbytes))
+        seg.sort()
+        nk = int(srow["num_k"])
+        if srow["kind"] == L.KIND_TOPK:
+            want = seg[::-1][:nk]
+        else:
+            kk = spec["k"][int(srow["k_offset"]):int(srow["k_offset"]) + nk].astype(np.int64)
+            want = seg[seg.size - kk]
+        g = got[int(srow["out_offset"]):int(srow["out_offset"]) + nk]
+        host_ok &= bool(np.array_equal(g.view(np.uint32), want.view(np.uint32)))
+    st = b.stats()
+    return {
+        "workload": f"{nseg} f32 vectors, lengths floor(2^U) U~U[10,22] (sum {N}), uniform or "
+                    "N(0,1) values, 1-64 uniform k or top-K (K log-uniform <= 4096) each",
+        "elements": N, "outputs": int(st["queries"]), "engine_segments": st["engine_segments"],
+        "direct_segments": st["direct_segments"], "chunks": st["chunks"],
+        "batched_ms": round(t_batch * 1e3, 3), "batched_gelem_per_s": round(N / t_batch / 1e9, 2),
+        "batched_hbm_frac": round(2 * N * 4 / t_batch / 1e9 / HBM_PEAK_GBS, 4),
+        "looped_ms": round(t_loop * 1e3, 2), "looped_gelem_per_s": round(N / t_loop / 1e9, 2),
+        "speedup_vs_looped": round(t_loop / t_batch, 2), "identical_to_looped": identical,
+        "host_sort_spot_check_64": host_ok, "graph": bool(st["graph"]),
+    }
+
+
+def serving_leg(pkg, rates, seconds: float):
+    """Served request streams (Poisson arrivals) over a pool of 64 resident
+    vectors (length log-uniform [2^12, 2^26], dtype f32:u32:f64 = 6:3:1,
+    uniform / normal / duplicate-heavy values), scratch budget 32 GB."""
+    S = pkg.serving
+    pkg.autotune.apply()
+    pool = S.make_pool(5000, 64)
+    bufs = S.upload_pool(pool)
+    host = S.sorted_copies(pool, bufs)
+    srv = pkg.Server(bufs, scratch_budget=32 << 30, max_inflight=4, max_batch=256)
+    out = []
+    for j, rate in enumerate(rates):
+        r = S.run(srv, pool, host, rate, seconds, seed=5000 + j)
+        print(f"# served {rate:.0f} req/s: {r}", file=sys.stderr, flush=True)
+        out.append(r)
+    srv.close()
+    return {"workload": "64 resident vectors, len log-uniform [2^12, 2^26], f32:u32:f64 = 6:3:1, "
+                        "uniform/normal/duplicate-heavy; 1-256 uniform k or top-K (K log-uniform "
+                        "<= 4096); Poisson arrivals; scratch budget 32 GB; 4 streams",
+            "pool_elements": int(pool["lengths"].sum()), "rates": out}
 
 
 def main():
@@ -164,13 +276,18 @@ def main():
     call_frac = 2 * n * esz / (step_ms * 1e-3) / 1e9 / HBM_PEAK_GBS
     value = n / (step_ms * 1e-3) / 1e9
 
+    # HBM bytes per launch of the dominant kernel, from the committed
+    # rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this workload (calibrated
+    # against a plain copy kernel), when they exist
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
             tt = json.load(f)
-        key = f"{a.kind}_{a.dtype}_2^{a.log2n}_{dom}"
-        traffic = tt.get(key)
+        rec = tt.get(f"{a.kind}_{a.dtype}_2^{a.log2n}_{k.size}k", {}).get("kernels", {})
+        kname = "k_count" if dom == "count" else "k_compact"
+        if kname in rec:
+            traffic = rec[kname]["bytes"]
 
     out = {
         "metric": METRIC,
@@ -209,6 +326,14 @@ def main():
     }
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(pkg, x, a.numk, a.cpu_seconds)
+    del d, x
+    if not a.no_batch:
+        print("# batched leg", file=sys.stderr, flush=True)
+        out["batched"] = batched_leg(pkg, L, a.batch_segments)
+    if not a.no_serve:
+        print("# serving leg", file=sys.stderr, flush=True)
+        out["serving"] = serving_leg(pkg, [float(r) for r in a.serve_rates.split(",")],
+                                     a.serve_seconds)
     print(json.dumps(out), flush=True)
 
 

@@ -8,7 +8,9 @@ The directory name holds a dash, so it is imported by path:
 from .api import (Batch, Context, DeviceArray, DeviceView, GGKSError,  # noqa: F401
                   RadixSelectWrapper, bucketMultiselectWrapper, generate, multiselect,
                   multiselect_async, randomizedTopkSelectWrapper, scratch_bytes,
-                  segment_table, sync, topk, tuning_get, tuning_reset, tuning_set)
+                  segment_table, Server, sync, topk, tuning_get, tuning_reset, tuning_set)
 from . import workloads  # noqa: F401
+from . import serving  # noqa: F401
+from . import autotune  # noqa: F401
 
 __version__ = "0.1"

@@ -31,9 +31,11 @@ EXPORTS = [
     "ggks_ctx_profile_read", "ggks_batch_create", "ggks_batch_scratch_bytes", "ggks_batch_run",
     "ggks_batch_sync", "ggks_batch_stream", "ggks_batch_info", "ggks_batch_destroy",
     "ggks_tuning_set", "ggks_tuning_get", "ggks_tuning_reset", "ggks_generate",
+    "ggks_server_create", "ggks_server_submit", "ggks_server_poll", "ggks_server_drain",
+    "ggks_server_run_trace", "ggks_server_stats_get", "ggks_server_destroy",
 ]
 BATCH_GRAPH = 1
-GEN_UNIFORM, GEN_NORMAL = 0, 1
+GEN_UNIFORM, GEN_NORMAL, GEN_LEVELS = 0, 1, 2
 NSTAGES = 8
 STAGES = ["queries", "sample", "lut", "count", "reduce", "findk", "compact", "finish"]
 
@@ -70,6 +72,29 @@ class BatchStats(ctypes.Structure):
                 ("scratch_bytes", ctypes.c_uint64), ("engine_segments", ctypes.c_uint32),
                 ("direct_segments", ctypes.c_uint32), ("chunks", ctypes.c_uint32),
                 ("graph", ctypes.c_uint32)]
+
+
+REQUEST_DTYPE = np.dtype([("offset", "<u8"), ("length", "<u8"), ("h_k", "<u8"), ("h_out", "<u8"),
+                          ("tag", "<u8"), ("num_k", "<u4"), ("kind", "<i4"), ("dtype", "<i4"),
+                          ("reserved", "<i4")])
+
+
+class Request(ctypes.Structure):
+    """ggks_request (include/ggks.h)."""
+    _fields_ = [("offset", ctypes.c_uint64), ("length", ctypes.c_uint64),
+                ("h_k", ctypes.c_void_p), ("h_out", ctypes.c_void_p), ("tag", ctypes.c_uint64),
+                ("num_k", ctypes.c_uint32), ("kind", ctypes.c_int32), ("dtype", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
+
+
+assert REQUEST_DTYPE.itemsize == ctypes.sizeof(Request)
+
+
+class ServerStats(ctypes.Structure):
+    """ggks_server_stats (include/ggks.h)."""
+    _fields_ = [(f, ctypes.c_uint64) for f in (
+        "submitted", "completed", "rejected", "batches", "elements", "max_pending", "pending",
+        "scratch_budget", "scratch_in_use", "peak_scratch_bytes")]
 
 
 _lib = None
@@ -126,6 +151,13 @@ def lib() -> ctypes.CDLL:
         "ggks_tuning_reset": ([], i32),
         "ggks_generate": ([vp, i32, i32, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64,
                            i64, i64], i32),
+        "ggks_server_create": ([ctypes.POINTER(vp), i32, vp, vp, sz, i32, i32], i32),
+        "ggks_server_submit": ([vp, ctypes.POINTER(Request)], i32),
+        "ggks_server_poll": ([vp, vp, vp, i32], i32),
+        "ggks_server_drain": ([vp], i32),
+        "ggks_server_run_trace": ([vp, vp, vp, i64, vp], i32),
+        "ggks_server_stats_get": ([vp, ctypes.POINTER(ServerStats)], i32),
+        "ggks_server_destroy": ([vp], i32),
     }
     for t in ("f32", "f64", "u32", "i32"):
         sig[f"ggks_bucket_multiselect_{t}"] = ([vp, i64, vp, ctypes.c_int32, vp, optp], i32)

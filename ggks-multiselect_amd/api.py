@@ -308,10 +308,11 @@ def tuning_reset() -> None:
 def generate(dtype, family: str, seed: int, n: int, start: int = 0, lo: int = 0, hi: int = 0,
              out: DeviceArray | None = None, offset: int = 0):
     """Fill a device array (new, or `out` from element `offset`) with n seeded
-    values: family "uniform" (floats in [0,1), integers in [lo, hi]) or
-    "normal"."""
+    values: family "uniform" (floats in [0,1), integers in [lo, hi]),
+    "normal", or "levels" (floats holding the integers lo..hi)."""
     d = out if out is not None else DeviceArray(n, dtype)
-    fam = {"uniform": _lib.GEN_UNIFORM, "normal": _lib.GEN_NORMAL}[family]
+    fam = {"uniform": _lib.GEN_UNIFORM, "normal": _lib.GEN_NORMAL,
+           "levels": _lib.GEN_LEVELS}[family]
     if offset < 0 or offset + n > d.n:
         raise ValueError("generate range outside the array")
     check(lib().ggks_generate(d.ptr + offset * d.dtype.itemsize, dtype_code(d.dtype), fam,
@@ -319,7 +320,86 @@ def generate(dtype, family: str, seed: int, n: int, start: int = 0, lo: int = 0,
     return d
 
 
+# ------------------------------------------------------------------ serving
+class Server:
+    """A ggks_server: continuous batching of select requests over vectors
+    resident in HBM.  `resident` maps a dtype to the DeviceArray holding its
+    vectors; requests name a vector by (dtype, offset, length)."""
+
+    def __init__(self, resident: dict, scratch_budget: int, max_inflight: int = 4,
+                 max_batch: int = 256, device: int = 0):
+        ptrs = (ctypes.c_void_p * 4)()
+        lens = (ctypes.c_uint64 * 4)()
+        self._resident = dict(resident)
+        for dt, arr in resident.items():
+            c = dtype_code(dt)
+            if np.dtype(dt) != arr.dtype:
+                raise TypeError("resident buffer dtype mismatch")
+            ptrs[c], lens[c] = arr.ptr, arr.n
+        p = ctypes.c_void_p()
+        check(lib().ggks_server_create(ctypes.byref(p), int(device), ptrs, lens,
+                                       int(scratch_budget), int(max_inflight), int(max_batch)))
+        self.handle = p.value
+        self._live = {}  # tag -> (k, out) kept alive until completion
+
+    def submit(self, dtype, offset: int, length: int, out: np.ndarray, k=None, K: int = 0,
+               kind: str = "bucket", tag: int = 0) -> None:
+        r = _lib.Request()
+        r.offset, r.length, r.tag = int(offset), int(length), int(tag)
+        r.dtype = dtype_code(dtype)
+        if kind == "topk":
+            r.kind, r.num_k, kk = _lib.KIND_TOPK, int(K), None
+        else:
+            kk = _k_array(k)
+            r.kind, r.num_k, r.h_k = _KIND_CODE[kind], kk.size, kk.ctypes.data
+        if out.dtype != np.dtype(dtype) or out.size < r.num_k or not out.flags.c_contiguous:
+            raise ValueError("output buffer must be a contiguous array of the dtype, num_k long")
+        r.h_out = out.ctypes.data
+        check(lib().ggks_server_submit(self.handle, ctypes.byref(r)))
+        self._live[int(tag)] = (kk, out)
+
+    def poll(self, max_done: int = 4096) -> list:
+        tags = np.zeros(max_done, np.uint64)
+        lat = np.zeros(max_done, np.float64)
+        n = lib().ggks_server_poll(self.handle, tags.ctypes.data, lat.ctypes.data, max_done)
+        if n < 0:
+            check(n)
+        for t in tags[:n]:
+            self._live.pop(int(t), None)
+        return list(zip(tags[:n].tolist(), lat[:n].tolist()))
+
+    def drain(self) -> None:
+        check(lib().ggks_server_drain(self.handle))
+
+    def run_trace(self, reqs: np.ndarray, arrival_s: np.ndarray) -> np.ndarray:
+        """Replay an arrival trace (REQUEST_DTYPE records whose h_k / h_out
+        point at live host buffers); returns latency per request (s, -1 if
+        refused)."""
+        reqs = np.ascontiguousarray(reqs, _lib.REQUEST_DTYPE)
+        arrival_s = np.ascontiguousarray(arrival_s, np.float64)
+        lat = np.full(reqs.size, -1.0)
+        check(lib().ggks_server_run_trace(self.handle, reqs.ctypes.data, arrival_s.ctypes.data,
+                                          reqs.size, lat.ctypes.data))
+        return lat
+
+    def stats(self) -> dict:
+        st = _lib.ServerStats()
+        check(lib().ggks_server_stats_get(self.handle, ctypes.byref(st)))
+        return {f: int(getattr(st, f)) for f, _ in st._fields_}
+
+    def close(self):
+        if self.handle:
+            lib().ggks_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 __all__ = ["DeviceArray", "DeviceView", "Context", "GGKSError", "multiselect", "topk",
            "multiselect_async", "scratch_bytes", "sync", "bucketMultiselectWrapper",
            "RadixSelectWrapper", "randomizedTopkSelectWrapper", "dtype_code", "Batch",
-           "segment_table", "tuning_set", "tuning_get", "tuning_reset", "generate"]
+           "segment_table", "tuning_set", "tuning_get", "tuning_reset", "generate", "Server"]

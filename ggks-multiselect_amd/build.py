@@ -15,8 +15,13 @@ ARCH = "gfx950"
 
 SOURCES = [os.path.join(HERE, "csrc", "ggks_select.hip")]
 HEADERS = [os.path.join(HERE, "csrc", h) for h in
-           ("ggks_keys.hpp", "ggks_types.hpp", "ggks_kernels.hpp", "ggks_plan.hpp")]
+           ("ggks_keys.hpp", "ggks_types.hpp", "ggks_kernels.hpp", "ggks_plan.hpp",
+            "ggks_batch.hpp", "ggks_serve.hpp", "ggks_gen.hpp")]
 OUT = os.path.join(HERE, "libggks.so")
+# C++ harness of the reference-named wrappers (include/ggks_compat.hpp):
+# host-only g++, linked against libggks.so
+HARNESS_SRC = os.path.join(ROOT, "tests", "cpp", "compat_harness.cpp")
+HARNESS = os.path.join(ROOT, "tests", "cpp", "compat_harness")
 
 FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
@@ -40,6 +45,15 @@ def build(verbose: bool = False) -> str:
     if verbose and err.strip():
         print(err, file=sys.stderr)
     os.replace(tmp, OUT)
+    hcmd = ["g++", "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"), HARNESS_SRC,
+            "-L" + HERE, "-lggks", "-Wl,-rpath,$ORIGIN/../../ggks-multiselect_amd",
+            "-o", HARNESS + ".tmp"]
+    if verbose:
+        print(" ".join(hcmd), flush=True)
+    r = subprocess.run(hcmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"g++ failed for the compat harness:\n{r.stderr}")
+    os.replace(HARNESS + ".tmp", HARNESS)
     return OUT
 
 

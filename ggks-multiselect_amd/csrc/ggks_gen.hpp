@@ -6,7 +6,9 @@
 // and integer-range families are bit-identical to the host's.  The normal
 // family is Box-Muller in double from two streams of the same seed, as on the
 // host, but with the device's log / sqrt / cos, so a value can differ from
-// the host's in its last bit.  -0.0 is canonicalised to +0.0.  Stands in for
+// the host's in its last bit.  The levels family gives floats with the
+// integer values lo..hi (heavy duplicates), as the host's int_range cast to
+// float.  -0.0 is canonicalised to +0.0.  Stands in for
 // the reference's cuRAND problem generators (generateProblems.cu:25-428),
 // whose streams are seeded by the clock and cannot be reproduced.
 #pragma once
@@ -24,7 +26,7 @@ __device__ __forceinline__ uint64_t splitmix_at(uint64_t seed, uint64_t i) {
   return z ^ (z >> 31);
 }
 
-enum GenFamily : int { GEN_UNIFORM = 0, GEN_NORMAL = 1 };
+enum GenFamily : int { GEN_UNIFORM = 0, GEN_NORMAL = 1, GEN_LEVELS = 2 };
 
 // dtype codes as in ggks.h: 0 f32, 1 f64, 2 u32, 3 i32.
 __global__ __launch_bounds__(256) void k_generate(void* __restrict__ dst, int dtype, int family,
@@ -40,7 +42,9 @@ __global__ __launch_bounds__(256) void k_generate(void* __restrict__ dst, int dt
       continue;
     }
     double x;
-    if (family == GEN_NORMAL) {
+    if (family == GEN_LEVELS) {  // integer values lo..hi as floats: heavy duplicates
+      x = (double)((uint64_t)lo + (z >> 16) % ((uint64_t)(hi - lo) + 1));
+    } else if (family == GEN_NORMAL) {
       const double u1 = ((double)(z >> 11) + 1.0) * 0x1p-53;
       const double u2 = (double)(splitmix_at(seed ^ 0x5DEECE66Dull, start + i) >> 11) * 0x1p-53;
       x = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);

@@ -111,9 +111,8 @@ __device__ uint32_t block_scan_array(uint32_t* a, uint32_t L, uint32_t* wsum) {
 
 // f(i, p[i]) for i = threadIdx.x, +kThreads, ... < n with 16 loads in flight
 // per thread (the loops that read another kernel's output are latency bound).
-template <typename T, typename F>
+template <uint32_t D = 16, typename T, typename F>
 __device__ __forceinline__ void strided_for(const T* __restrict__ p, uint32_t n, F&& f) {
-  constexpr uint32_t D = 16;
   uint32_t i = threadIdx.x;
   for (; i + (D - 1) * kThreads < n; i += D * kThreads) {
     T v[D];
@@ -186,23 +185,23 @@ struct BucketFn {
     split(key, p, t);
     return finish(lut[p], t);
   }
-  // candidate cell: prefix and the top sc_wid bits of the sub-field (no LUT)
+  // candidate cell: the prefix and the top sc_wid bits of the sub-field,
+  // (p << sc_wid) | c -- one bit field of low, as the sub-field's top bits sit
+  // right below the prefix (sc_off + sc_wid = poff); no LUT, no multiply
   uint32_t sc_off, sc_wid;
   __device__ __forceinline__ uint32_t cell(KT key) const {
     KT low = (key > kb ? key : kb) - kb;
     low = low < lmax ? low : lmax;
-    uint32_t p, c;
-    if (sizeof(KT) == 4) {
-      p = __builtin_amdgcn_ubfe((uint32_t)low, poff, pwid);
-      c = __builtin_amdgcn_ubfe((uint32_t)low, sc_off, sc_wid);
-    } else {
-      const uint64_t l64 = (uint64_t)low;
-      p = (uint32_t)(l64 >> poff) & ((1u << pwid) - 1u);
-      c = (uint32_t)(l64 >> sc_off) & ((1u << sc_wid) - 1u);
-    }
-    return (p << kCellBits) | c;
+    if (sizeof(KT) == 4) return __builtin_amdgcn_ubfe((uint32_t)low, sc_off, sc_wid + pwid);
+    return (uint32_t)((uint64_t)low >> sc_off) & ((1u << (sc_wid + pwid)) - 1u);
   }
 };
+
+// Bit c of an LDS bitmap.  v_bfe_u32 (and the intrinsic's definition) take
+// the bit offset modulo 32, so c needs no mask.
+__device__ __forceinline__ uint32_t bitmap_test(const uint32_t* bm, uint32_t c) {
+  return __builtin_amdgcn_ubfe(bm[c >> 5], c, 1u);
+}
 
 // Buckets of N keys with the N look-ups issued back to back, so a wavefront
 // waits on LDS once per batch rather than once per key.
@@ -352,27 +351,53 @@ __global__ void k_topk_queries(uint32_t K, uint64_t n, Query* __restrict__ q) {
   q[i].out = i;
 }
 
-// Seeded random sample of each segment: grid (segments, sample blocks).
+// Seeded random sample of each segment: grid (segments, sample blocks of
+// 1024; every sample count is a multiple of 1024).  Each work-group also
+// writes the min and max key of its 1024 samples, so the LUT kernel reads 64
+// pairs instead of scanning the sample for its range.
 template <typename KT, int C>
 __global__ __launch_bounds__(kThreads) void k_sample(const KT* __restrict__ src,
                                                      const SegDesc* __restrict__ segs,
-                                                     KT* __restrict__ samples, uint64_t seed) {
+                                                     KT* __restrict__ samples,
+                                                     KT* __restrict__ minmax, uint64_t seed) {
+  __shared__ KT red_lo[kThreads / 64], red_hi[kThreads / 64];
   const SegDesc& sd = segs[blockIdx.x];
-  if (sd.mode != MODE_SAMPLE) return;
-  uint32_t i = blockIdx.y * kThreads + threadIdx.x;
-  if (i >= sd.nsample) return;
+  if (sd.mode != MODE_SAMPLE || blockIdx.y * kThreads >= sd.nsample) return;  // uniform
+  const uint32_t i = blockIdx.y * kThreads + threadIdx.x;
   uint64_t z = seed + 0x9E3779B97F4A7C15ull * ((uint64_t)blockIdx.x * 0x100000001ull + i + 1);
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
   z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
   z ^= z >> 31;
-  uint64_t pos = __umul64hi(z, sd.len);
-  samples[sd.sample_off + i] = to_key<C>(src[sd.data_off + pos]);
+  const uint64_t pos = __umul64hi(z, sd.len);
+  const KT k = to_key<C>(src[sd.data_off + pos]);
+  samples[sd.sample_off + i] = k;
+  KT lo = k, hi = k;
+  for (int d = 32; d >= 1; d >>= 1) {
+    const KT a = __shfl_xor(lo, d, 64), b = __shfl_xor(hi, d, 64);
+    lo = a < lo ? a : lo;
+    hi = b > hi ? b : hi;
+  }
+  if (lane_id() == 0) {
+    red_lo[threadIdx.x >> 6] = lo;
+    red_hi[threadIdx.x >> 6] = hi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (uint32_t w = 1; w < kThreads / 64; ++w) {
+      lo = red_lo[w] < lo ? red_lo[w] : lo;
+      hi = red_hi[w] > hi ? red_hi[w] : hi;
+    }
+    KT* mm = minmax + 2 * (sd.sample_off / kThreads + blockIdx.y);
+    mm[0] = lo;
+    mm[1] = hi;
+  }
 }
 
 // Build one segment's bucket function (one block per segment).
 template <typename KT>
 __global__ __launch_bounds__(kThreads) void k_lut(const SegDesc* __restrict__ segs,
                                                   const KT* __restrict__ samples,
+                                                  const KT* __restrict__ minmax,
                                                   SegState* __restrict__ states,
                                                   uint32_t* __restrict__ lut_pool) {
   __shared__ uint32_t cnt[1u << kMaxPrefixBits];
@@ -387,10 +412,12 @@ __global__ __launch_bounds__(kThreads) void k_lut(const SegDesc* __restrict__ se
 
   if (sd.mode == MODE_SAMPLE) {
     KT lo = KeyBits<KT>::MAX, hi = 0;
-    strided_for(samples + sd.sample_off, sd.nsample, [&](uint32_t, KT k) {
-      lo = k < lo ? k : lo;
-      hi = k > hi ? k : hi;
-    });
+    const uint32_t npairs = sd.nsample / kThreads;  // <= 64
+    if (tid < npairs) {
+      const KT* mm = minmax + 2 * (sd.sample_off / kThreads + tid);
+      lo = mm[0];
+      hi = mm[1];
+    }
     for (int d = 32; d >= 1; d >>= 1) {
       KT a = __shfl_xor(lo, d, 64), b = __shfl_xor(hi, d, 64);
       lo = a < lo ? a : lo;
@@ -441,7 +468,8 @@ __global__ __launch_bounds__(kThreads) void k_lut(const SegDesc* __restrict__ se
   } else {
     for (uint32_t p = tid; p < P; p += kThreads) cnt[p] = 0;
     __syncthreads();
-    strided_for(samples + sd.sample_off, sd.nsample, [&](uint32_t, KT k) {
+    // the whole sample in one round of loads (64 per thread for 4-byte keys)
+    strided_for<64 * 4 / sizeof(KT)>(samples + sd.sample_off, sd.nsample, [&](uint32_t, KT k) {
       atomicAdd(&cnt[(uint32_t)((k - kmin) >> s)], 1u);
     });
     __syncthreads();
@@ -809,13 +837,17 @@ constexpr uint32_t kWaves = kThreads / 64;
 constexpr int kRound = 4;                   // 16-byte loads per lane per round
 constexpr uint32_t kCompactCursors = 1024;  // LDS cursors; children beyond use global atomics
 
+// candidate-cell bitmap of the largest prefix table, in static LDS at a
+// fixed address (the word address of a cell then needs no base add)
+constexpr uint32_t kCandWords = (1u << kMaxPrefixBits) << (kCellBits - 5);
+
 template <typename KT>
 __host__ __device__ constexpr size_t compact_lds_bytes(uint32_t pbits, uint32_t bmax) {
-  // candidate bitmap, LUT, (bits, child base) per word, cursors, the round's
-  // keys of every lane (64 B each) and the staged candidate indices
-  return ((size_t)1 << (pbits + kCellBits - 3)) + (size_t)(1u << pbits) * 4 +
-         (size_t)((bmax + 31) / 32) * 8 + (size_t)kCompactCursors * 4 +
-         (size_t)kThreads * kRound * 16 + (size_t)kThreads * kRound * (16 / sizeof(KT)) * 2;
+  // dynamic part: LUT, (bits, child base) per word, cursors, the round's keys
+  // of every lane (64 B each) and the staged candidate indices
+  return (size_t)(1u << pbits) * 4 + (size_t)((bmax + 31) / 32) * 8 +
+         (size_t)kCompactCursors * 4 + (size_t)kThreads * kRound * 16 +
+         (size_t)kThreads * kRound * (16 / sizeof(KT)) * 2;
 }
 
 template <typename KT, int C>
@@ -831,6 +863,7 @@ __global__ __launch_bounds__(kThreads) void k_compact(const KT* __restrict__ src
                                                       const Child* __restrict__ children,
                                                       KT* __restrict__ red) {
   extern __shared__ __align__(16) uint32_t lds[];
+  __shared__ uint32_t cbm[kCandWords];
   const Chunk ch = chunks[blockIdx.x];
   const SegDesc& sd = segs[ch.seg];
   const SegState st = states[ch.seg];
@@ -841,8 +874,7 @@ __global__ __launch_bounds__(kThreads) void k_compact(const KT* __restrict__ src
   const uint32_t nwords = (st.B + 31) / 32;
   const uint32_t cwords = st.P << (kCellBits - 5);
   char* base = reinterpret_cast<char*>(lds);
-  uint32_t* cbm = reinterpret_cast<uint32_t*>(base);
-  uint32_t* lut = reinterpret_cast<uint32_t*>(base + ((size_t)1 << (sd.pbits + kCellBits - 3)));
+  uint32_t* lut = reinterpret_cast<uint32_t*>(base);
   uint2* bw = reinterpret_cast<uint2*>(lut + (1u << sd.pbits));  // (bits, child base)
   uint32_t* cur = reinterpret_cast<uint32_t*>(bw + (sd.bmax + 31) / 32);
   uint4* kbuf = reinterpret_cast<uint4*>(cur + kCompactCursors);  // kRound per lane
@@ -912,7 +944,7 @@ __global__ __launch_bounds__(kThreads) void k_compact(const KT* __restrict__ src
 #pragma unroll
     for (int i = 0; i < N; ++i) c[i] = bk.cell(to_key<C>(k[i]));
 #pragma unroll
-    for (int i = 0; i < N; ++i) mask |= ((cbm[c[i] >> 5] >> (c[i] & 31u)) & 1u) << i;
+    for (int i = 0; i < N; ++i) mask |= bitmap_test(cbm, c[i]) << i;
     mask &= okmask;
     const uint32_t cl = __popc(mask);
     const uint32_t inc = wave_incl_scan(cl);
@@ -947,6 +979,136 @@ __global__ __launch_bounds__(kThreads) void k_copy16(const uint4* __restrict__ s
   const uint64_t stride = (uint64_t)gridDim.x * kThreads;
   for (uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x; i < nv; i += stride)
     dst[i] = stream_load(&src[i]);
+}
+
+// Children with few ranks are finished by one wavefront each (k_finish_wave);
+// the others by a whole work-group (k_finish).
+constexpr uint32_t kWaveRanks = 4;
+constexpr uint64_t kWaveMaxLen = 1ull << 15;
+__device__ __forceinline__ bool wave_child(const Child& c) {
+  return c.q_cnt <= kWaveRanks && c.len <= kWaveMaxLen;
+}
+
+// Visit keys p[0, len) of one child with a wavefront: 16-byte loads for the
+// aligned body (eight in flight per lane), scalar head and tail.
+template <typename KT, typename F>
+__device__ __forceinline__ void wave_for_each(const KT* __restrict__ p, uint64_t len, uint32_t lane,
+                                              F&& f) {
+  constexpr uint32_t VEC = 16 / sizeof(KT);
+  uint64_t mis = ((uintptr_t)p & 15u) / sizeof(KT);
+  uint64_t head = mis ? (VEC - mis) : 0;
+  if (head > len) head = len;
+  if (lane < head) f(p[lane]);
+  const uint64_t nvec = (len - head) / VEC;
+  const uint4* __restrict__ pv = reinterpret_cast<const uint4*>(p + head);
+  auto apply = [&](const uint4& v) {
+    if (sizeof(KT) == 4) {
+      f((KT)v.x); f((KT)v.y); f((KT)v.z); f((KT)v.w);
+    } else {
+      f((KT)(((uint64_t)v.y << 32) | v.x));
+      f((KT)(((uint64_t)v.w << 32) | v.z));
+    }
+  };
+  uint64_t i = lane;
+  for (; i + 7 * 64 < nvec; i += 8 * 64) {
+    uint4 v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = pv[i + u * 64];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) apply(v[u]);
+  }
+  for (; i < nvec; i += 64) apply(pv[i]);
+  const uint64_t done = head + nvec * VEC;
+  if (done + lane < len) f(p[done + lane]);
+}
+
+// Finisher for children with at most kWaveRanks ranks: one wavefront per
+// child, MSD radix select on 8-bit digits of (key - lo) with a private
+// 256-bin LDS histogram per wavefront; the keys are re-read (from L2) each
+// digit pass instead of being staged.  No work-group barriers, so every
+// child of a launch runs at once.
+template <typename KT, int C>
+__global__ __launch_bounds__(kThreads) void k_finish_wave(const KT* __restrict__ input,
+                                                          const KT* __restrict__ red,
+                                                          const Child* __restrict__ children,
+                                                          uint32_t nchildren,
+                                                          const Query* __restrict__ cq,
+                                                          KT* __restrict__ out) {
+  constexpr uint32_t W = KeyBits<KT>::W;
+  __shared__ uint32_t hist_all[kThreads / 64][256];
+  const uint32_t lane = lane_id();
+  const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  uint32_t* h = hist_all[wave];
+  auto wave_sync = [] {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  };
+  for (uint32_t ci = blockIdx.x * (kThreads / 64) + wave; ci < nchildren;
+       ci += gridDim.x * (kThreads / 64)) {
+    const Child c = children[ci];
+    if (c.q_cnt == 0 || !wave_child(c)) continue;
+    const KT* srcp = (c.src ? red : input) + c.start;
+    KT lo = (KT)c.lo, hi = (KT)c.hi;
+    if (c.need_range) {
+      KT a = KeyBits<KT>::MAX, b = 0;
+      wave_for_each(srcp, c.len, lane, [&](KT raw) {
+        const KT k = to_key<C>(raw);
+        a = k < a ? k : a;
+        b = k > b ? k : b;
+      });
+      for (int d = 32; d >= 1; d >>= 1) {
+        const KT x = __shfl_xor(a, d, 64), y = __shfl_xor(b, d, 64);
+        a = x < a ? x : a;
+        b = y > b ? y : b;
+      }
+      lo = a;
+      hi = b;
+    }
+    const int nbits = bitlen<KT>((KT)(hi - lo));
+    for (uint32_t qi = 0; qi < c.q_cnt; ++qi) {
+      const Query q = cq[c.q_off + qi];
+      uint32_t r1 = q.rank - 1;  // rank among the keys with the current prefix
+      KT pre = 0;
+      int rem = nbits;
+      while (rem > 0) {
+        const int d = rem < 8 ? rem : 8;
+        const int shift = rem - d;
+        const int hshift = shift + d;
+        const uint32_t dmask = (1u << d) - 1u;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) h[lane * 4 + j] = 0;
+        wave_sync();
+        wave_for_each(srcp, c.len, lane, [&](KT raw) {
+          const KT rel = (KT)(to_key<C>(raw) - lo);
+          const KT top = hshift >= (int)W ? KT(0) : (KT)(rel >> hshift);
+          if (top == pre) atomicAdd(&h[(uint32_t)(rel >> shift) & dmask], 1u);
+        });
+        wave_sync();
+        // bins 4 lane .. 4 lane + 3; exclusive starts across the wavefront
+        const uint32_t v0 = h[lane * 4], v1 = h[lane * 4 + 1], v2 = h[lane * 4 + 2],
+                       v3 = h[lane * 4 + 3];
+        const uint32_t sum = v0 + v1 + v2 + v3;
+        const uint32_t e0 = wave_incl_scan(sum) - sum;
+        const uint32_t e1 = e0 + v0, e2 = e1 + v1, e3 = e2 + v2;
+        // the bin holding rank r1: start <= r1 < start + count (exactly one)
+        int hit = -1;
+        uint32_t start = 0;
+        if (r1 >= e0 && r1 < e1) { hit = 0; start = e0; }
+        if (r1 >= e1 && r1 < e2) { hit = 1; start = e1; }
+        if (r1 >= e2 && r1 < e3) { hit = 2; start = e2; }
+        if (r1 >= e3 && r1 < e3 + v3) { hit = 3; start = e3; }
+        const uint64_t who = __ballot(hit >= 0);
+        const int src = __ffsll((unsigned long long)who) - 1;
+        const uint32_t dg = (uint32_t)__shfl(lane * 4 + hit, src, 64);
+        r1 -= (uint32_t)__shfl(start, src, 64);
+        pre = (KT)((pre << d) | (KT)dg);
+        rem = shift;
+        wave_sync();
+      }
+      if (lane == 0) out[q.out] = from_key<C>((KT)(lo + pre));
+    }
+  }
 }
 
 // Finisher: exact multi-rank radix select inside each child.
@@ -984,7 +1146,7 @@ __global__ __launch_bounds__(kThreads) void k_finish(const KT* __restrict__ inpu
 
   for (uint32_t ci = blockIdx.x; ci < nchildren; ci += gridDim.x) {
     const Child c = children[ci];
-    if (c.q_cnt == 0) continue;
+    if (c.q_cnt == 0 || wave_child(c)) continue;
     const KT* srcp = (c.src ? red : input) + c.start;
     const bool inl = c.len <= CAP;
     KT lo = (KT)c.lo, hi = (KT)c.hi;

@@ -47,7 +47,7 @@ struct Plan {
   uint64_t seed = 0;
   // arena offsets (bytes)
   size_t off_tables = 0, off_segs = 0, off_chunks = 0, off_items = 0, tables_end = 0;
-  size_t off_states = 0, off_samples = 0, off_lut = 0, off_rows = 0, off_tot = 0;
+  size_t off_states = 0, off_samples = 0, off_minmax = 0, off_lut = 0, off_rows = 0, off_tot = 0;
   size_t off_bits = 0, off_wbase = 0, off_cand = 0, off_q = 0, off_qb = 0, off_qr = 0;
   size_t off_children = 0, off_cq = 0, off_red = 0, total = 0;
   size_t off_kvals = 0, off_dchildren = 0, off_dqueries = 0;  // batch / direct tables
@@ -172,6 +172,7 @@ inline void plan_single(Plan& p, int kind, int dtype, uint64_t n, uint32_t nq,
   p.tables_end = off;
   p.off_states = take(p.segs.size() * sizeof(SegState));
   p.off_samples = take((size_t)std::max<uint32_t>(ns, 1) * eb);
+  p.off_minmax = take((size_t)std::max<uint32_t>(ns / 1024, 1) * 2 * eb);
   p.off_lut = take((size_t)(1u << pb) * 4);
   p.off_rows = take((size_t)G * bmax * 4);
   p.off_tot = take((size_t)bmax * 4);
@@ -369,6 +370,7 @@ inline void plan_batch(Plan& p, int dtype, const std::vector<BatchSeg>& in, cons
   p.tables_end = off;
   p.off_states = take(nseg * sizeof(SegState));
   p.off_samples = take((size_t)std::max<uint32_t>(sam, 1) * eb);
+  p.off_minmax = take((size_t)std::max<uint32_t>(sam / 1024, 1) * 2 * eb);
   p.off_lut = take((size_t)lutc * 4);
   p.off_rows = take((size_t)rowc * 4);
   p.off_tot = take((size_t)totc * 4);

@@ -7,6 +7,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
+#include <deque>
+#include <map>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -172,6 +175,18 @@ static void level_attrs(const Plan&) {
   (void)once;
 }
 
+// Both finishers over a child list: one wavefront per child with few ranks,
+// one work-group per child otherwise (each kernel skips the other's).
+template <typename KT, int C>
+static void finish(hipStream_t s, const KT* d_src, const KT* red, const Child* kids, uint32_t n,
+                   const Query* cq, KT* d_out, uint32_t max_groups) {
+  const uint32_t wgrid = std::min<uint32_t>((n + kThreads / 64 - 1) / (kThreads / 64), 8192);
+  hipLaunchKernelGGL((k_finish_wave<KT, C>), dim3(wgrid), dim3(kThreads), 0, s, d_src, red, kids,
+                     n, cq, d_out);
+  hipLaunchKernelGGL((k_finish<KT, C>), dim3(std::min<uint32_t>(n, max_groups)), dim3(kThreads),
+                     kFinishLdsTotal, s, d_src, red, kids, n, cq, d_out);
+}
+
 // The level pipeline over every engine segment of the plan, then the
 // finisher over the plan's direct children (whole small segments).
 template <typename KT, int C>
@@ -186,6 +201,7 @@ static int enqueue_level(const Exec& x, const Plan& p, const KT* d_src, KT* d_ou
     const Chunk* chunks = reinterpret_cast<const Chunk*>(A + p.off_chunks);
     const ReduceItem* items = reinterpret_cast<const ReduceItem*>(A + p.off_items);
     KT* samples = reinterpret_cast<KT*>(A + p.off_samples);
+    KT* minmax = reinterpret_cast<KT*>(A + p.off_minmax);
     uint32_t* lut = reinterpret_cast<uint32_t*>(A + p.off_lut);
     uint32_t* rows = reinterpret_cast<uint32_t*>(A + p.off_rows);
     uint32_t* tot = reinterpret_cast<uint32_t*>(A + p.off_tot);
@@ -203,10 +219,12 @@ static int enqueue_level(const Exec& x, const Plan& p, const KT* d_src, KT* d_ou
     stage(x, 1);
     if (p.any_sample) {
       dim3 g(nseg, (p.max_nsample + kThreads - 1) / kThreads);
-      hipLaunchKernelGGL((k_sample<KT, C>), g, dim3(kThreads), 0, s, d_src, segs, samples, p.seed);
+      hipLaunchKernelGGL((k_sample<KT, C>), g, dim3(kThreads), 0, s, d_src, segs, samples, minmax,
+                         p.seed);
     }
     stage(x, 2);
-    hipLaunchKernelGGL((k_lut<KT>), dim3(nseg), dim3(kThreads), 0, s, segs, samples, states, lut);
+    hipLaunchKernelGGL((k_lut<KT>), dim3(nseg), dim3(kThreads), 0, s, segs, samples, minmax, states,
+                       lut);
     stage(x, 3);
     hipLaunchKernelGGL((k_count<KT, C>), dim3(nchunks), dim3(kThreads),
                        ((size_t)p.bmax_max + (1u << p.pbits_max)) * 4, s, d_src, segs, states,
@@ -225,9 +243,7 @@ static int enqueue_level(const Exec& x, const Plan& p, const KT* d_src, KT* d_ou
                        compact_lds_bytes<KT>(p.pbits_max, p.bmax_max), s, d_src, segs, states,
                        chunks, lut, rows, bits, wbase, cand, kids, red);
     stage(x, 7);
-    const uint32_t fgrid = std::min<uint32_t>(p.nq, 2048);
-    hipLaunchKernelGGL((k_finish<KT, C>), dim3(fgrid), dim3(kThreads), kFinishLdsTotal, s, d_src,
-                       red, kids, p.nq, cq, d_out);
+    finish<KT, C>(s, d_src, red, kids, p.nq, cq, d_out, 2048);
   } else {
     for (int st = 1; st < GGKS_NSTAGES; ++st) stage(x, st);
   }
@@ -235,9 +251,7 @@ static int enqueue_level(const Exec& x, const Plan& p, const KT* d_src, KT* d_ou
   if (ndirect > 0) {
     const Child* dk = reinterpret_cast<const Child*>(A + p.off_dchildren);
     const Query* dq = reinterpret_cast<const Query*>(A + p.off_dqueries);
-    hipLaunchKernelGGL((k_finish<KT, C>), dim3(std::min<uint32_t>(ndirect, 4096)),
-                       dim3(kThreads), kFinishLdsTotal, s, d_src, (const KT*)nullptr, dk, ndirect,
-                       dq, d_out);
+    finish<KT, C>(s, d_src, (const KT*)nullptr, dk, ndirect, dq, d_out, 4096);
   }
   HIP_TRY(hipGetLastError());
   return GGKS_OK;
@@ -382,6 +396,7 @@ static int select_one(ggks_ctx* c, int kind, int dtype, const void* d_vec, int64
 }
 
 #include "ggks_batch.hpp"
+#include "ggks_serve.hpp"
 
 }  // namespace ggks
 
@@ -662,8 +677,10 @@ int ggks_tuning_reset(void) {
 int ggks_generate(void* d_ptr, int dtype, int family, uint64_t seed, uint64_t start, uint64_t n,
                   int64_t lo, int64_t hi) {
   if (!d_ptr && n) return fail(GGKS_EINVAL, "null pointer");
-  if (dtype < 0 || dtype > 3 || family < 0 || family > 1)
+  if (dtype < 0 || dtype > 3 || family < 0 || family > 2)
     return fail(GGKS_EINVAL, "bad dtype %d or family %d", dtype, family);
+  if (family == 2 && (hi < lo || lo < 0 || hi > 0xFFFFFFFFll))
+    return fail(GGKS_EINVAL, "bad levels range [%lld, %lld]", (long long)lo, (long long)hi);
   if ((dtype == GGKS_U32 || dtype == GGKS_I32) &&
       (hi < lo || (dtype == GGKS_U32 && (lo < 0 || hi > 0xFFFFFFFFll)) ||
        (dtype == GGKS_I32 && (lo < INT32_MIN || hi > INT32_MAX))))
@@ -676,5 +693,97 @@ int ggks_generate(void* d_ptr, int dtype, int family, uint64_t seed, uint64_t st
   HIP_TRY(hipDeviceSynchronize());
   return GGKS_OK;
 }
+
+// ---------------------------------------------------------------- serving
+int ggks_server_create(ggks_server** out, int device, const void* const* resident,
+                       const uint64_t* resident_len, size_t scratch_budget, int max_inflight,
+                       int max_batch) {
+  if (!out || !resident || !resident_len) return fail(GGKS_EINVAL, "null argument");
+  if (scratch_budget < (1u << 20) || max_inflight < 1 || max_inflight > 16 || max_batch < 1)
+    return fail(GGKS_EINVAL, "bad server limits");
+  HIP_TRY(hipSetDevice(device));
+  ggks_server* s = new ggks_server();
+  s->device = device;
+  for (int d = 0; d < 4; ++d) {
+    s->resident[d] = static_cast<const char*>(resident[d]);
+    s->resident_len[d] = resident[d] ? resident_len[d] : 0;
+  }
+  s->budget = align_up(scratch_budget);
+  s->max_batch = (uint32_t)max_batch;
+  if (hipMalloc(&s->pool, s->budget) != hipSuccess) {
+    (void)hipGetLastError();
+    s->pool = nullptr;
+    serve_destroy(s);
+    return fail(GGKS_ENOMEM, "cannot allocate the %zu-byte scratch budget", scratch_budget);
+  }
+  s->alloc.reset(s->budget);
+  s->st.scratch_budget = s->budget;
+  s->slots.resize((size_t)max_inflight);
+  for (ServeSlot& sl : s->slots) {
+    if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess) {
+      const hipError_t e = hipGetLastError();
+      serve_destroy(s);
+      return fail(GGKS_EDEVICE, "cannot create server streams: %s", hipGetErrorString(e));
+    }
+  }
+  *out = s;
+  return GGKS_OK;
+}
+
+int ggks_server_submit(ggks_server* s, const ggks_request* r) {
+  if (!s) return fail(GGKS_EINVAL, "null server");
+  HIP_TRY(hipSetDevice(s->device));
+  int rc = serve_submit(s, r, now_s());
+  if (rc) return rc;
+  size_t nc = 0;
+  return serve_step(s, nc);
+}
+
+int ggks_server_poll(ggks_server* s, uint64_t* tags, double* latency_s, int max_done) {
+  if (!s || max_done < 0 || (max_done > 0 && (!tags || !latency_s)))
+    return fail(GGKS_EINVAL, "bad poll arguments");
+  HIP_TRY(hipSetDevice(s->device));
+  size_t nc = 0;
+  int rc = serve_step(s, nc);
+  if (rc) return rc;
+  int k = 0;
+  while (k < max_done && !s->finished.empty()) {
+    tags[k] = s->finished.front().first;
+    latency_s[k] = s->finished.front().second;
+    s->finished.pop_front();
+    ++k;
+  }
+  return k;
+}
+
+int ggks_server_drain(ggks_server* s) {
+  if (!s) return fail(GGKS_EINVAL, "null server");
+  HIP_TRY(hipSetDevice(s->device));
+  size_t nc = 0;
+  while (!serve_idle(s)) {
+    int rc = serve_step(s, nc);
+    if (rc) return rc;
+  }
+  return GGKS_OK;
+}
+
+int ggks_server_run_trace(ggks_server* s, const ggks_request* reqs, const double* arrival_s,
+                          int64_t n, double* latency_s) {
+  if (!s || n < 0 || (n > 0 && (!reqs || !arrival_s || !latency_s)))
+    return fail(GGKS_EINVAL, "bad trace arguments");
+  HIP_TRY(hipSetDevice(s->device));
+  return serve_run_trace(s, reqs, arrival_s, n, latency_s);
+}
+
+int ggks_server_stats_get(ggks_server* s, ggks_server_stats* out) {
+  if (!s || !out) return fail(GGKS_EINVAL, "null argument");
+  *out = s->st;
+  out->pending = s->pending.size();
+  out->scratch_in_use = s->alloc.used();
+  return GGKS_OK;
+}
+
+int ggks_server_destroy(ggks_server* s) { return serve_destroy(s); }
 
 }  // extern "C"

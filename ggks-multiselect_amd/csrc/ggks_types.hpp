@@ -73,8 +73,8 @@ struct SegDesc {          // planned on the host, read-only on the device
 //   t   = ((low >> rsh) & (2^twid - 1)) << lsh   the low s bits of low,
 //                                           normalised to 24 bits
 //   b   = base[p] + (t * nb[p]) >> 24       (one 24-bit multiply-high)
-// and the candidate cell of a key is (p << kCellBits) | (top bits of the
-// sub-field), monotone in the key like the bucket.
+// and the candidate cell of a key is (p << sc_wid) | (top sc_wid <= kCellBits
+// bits of the sub-field), monotone in the key like the bucket.
 struct SegState {         // written on the device by the LUT kernel
   uint64_t kb;            // base key
   uint64_t kmax;          // largest sampled key (information)

@@ -229,6 +229,63 @@ int ggks_tuning_set(int dtype, int log2_len, int buckets_log2, int chunk_log2, i
 int ggks_tuning_get(int dtype, int log2_len, int *buckets_log2, int *chunk_log2, int *direct);
 int ggks_tuning_reset(void);
 
+/* ---- serving loop ----------------------------------------------------
+ * No reference counterpart.  A server keeps select requests over vectors
+ * resident in HBM and runs them by continuous batching: whenever one of its
+ * max_inflight streams is free, the oldest pending request and the later
+ * pending requests of the same dtype (up to max_batch) are planned as one
+ * batch and admitted if the batch's scratch fits in the unused part of the
+ * scratch budget (the group is halved until it does; otherwise it waits for
+ * running batches to release memory).  Batches on different streams overlap
+ * on the device.  Every output equals the single-vector entry point's for
+ * that request.  One host thread drives a server; scheduling happens inside
+ * submit, poll, drain and run_trace. */
+typedef struct ggks_request {
+  uint64_t offset;     /* first element of the vector in its dtype's resident buffer */
+  uint64_t length;     /* elements, 1 <= length < 2^32 */
+  const uint32_t *h_k; /* multiselect: num_k k values, copied at submission; top-K: unused */
+  void *h_out;         /* host outputs (num_k values of the dtype), written on completion */
+  uint64_t tag;        /* reported by ggks_server_poll */
+  uint32_t num_k;      /* multiselect: number of k values; top-K: K */
+  int32_t kind;        /* GGKS_KIND_BUCKET, GGKS_KIND_RADIX or GGKS_KIND_TOPK */
+  int32_t dtype;       /* selects the resident buffer */
+  int32_t reserved;
+} ggks_request;
+
+typedef struct ggks_server_stats {
+  uint64_t submitted, completed, rejected, batches;
+  uint64_t elements;           /* sum of lengths of completed requests */
+  uint64_t max_pending;        /* deepest admission queue seen */
+  uint64_t pending;            /* queue depth now */
+  uint64_t scratch_budget;     /* bytes */
+  uint64_t scratch_in_use;     /* bytes now */
+  uint64_t peak_scratch_bytes; /* most bytes in use after an admission */
+} ggks_server_stats;
+
+typedef struct ggks_server ggks_server;
+
+/* resident[d] / resident_len[d]: device buffer and element count holding the
+ * vectors of dtype d (NULL for a dtype not served).  scratch_budget: device
+ * bytes the server allocates once and admits batches into.  max_inflight:
+ * concurrent batches (streams), 1..16. */
+int ggks_server_create(ggks_server **server, int device, const void *const *resident,
+                       const uint64_t *resident_len, size_t scratch_budget, int max_inflight,
+                       int max_batch);
+/* Queue a request (validated now; GGKS_ENOMEM if it alone exceeds the budget). */
+int ggks_server_submit(ggks_server *server, const ggks_request *req);
+/* Schedule, then report up to max_done completed requests (tag and latency in
+ * seconds from submission); returns how many were written. */
+int ggks_server_poll(ggks_server *server, uint64_t *tags, double *latency_s, int max_done);
+/* Run until nothing is pending or in flight. */
+int ggks_server_drain(ggks_server *server);
+/* Replay an open-loop arrival trace (arrival_s ascending, seconds from the
+ * call): request i is submitted at arrival_s[i]; latency_s[i] = completion
+ * seen - arrival_s[i], or -1 if the request was refused. */
+int ggks_server_run_trace(ggks_server *server, const ggks_request *reqs, const double *arrival_s,
+                          int64_t n, double *latency_s);
+int ggks_server_stats_get(ggks_server *server, ggks_server_stats *stats);
+int ggks_server_destroy(ggks_server *server);
+
 /* ---- seeded synthetic inputs on the device ------------------------------
  * Replaces the reference's cuRAND problem generators (generateProblems.cu:
  * 25-428, clock-seeded there).  Element i is drawn from output start + i of
@@ -236,7 +293,8 @@ int ggks_tuning_reset(void);
  * grid of [0,1), f64 on the 2^-53 grid, u32/i32 uniform in [lo, hi]
  * (bit-identical to the Python package's host generators); family 1 =
  * normal(0,1) by Box-Muller in double (f32/f64; the device's libm may differ
- * from the host's in the last bit).  Synchronous. */
+ * from the host's in the last bit); family 2 = levels: f32/f64 holding the
+ * integers lo..hi (0 <= lo <= hi < 2^32), heavy duplicates.  Synchronous. */
 int ggks_generate(void *d_ptr, int dtype, int family, uint64_t seed, uint64_t start, uint64_t n,
                   int64_t lo, int64_t hi);
 

@@ -186,3 +186,14 @@ def test_device_copy_plumbing(pkg):
         h = y.to_host()
         m = nbytes // 4
         assert np.array_equal(h[:m], x[:m]) and not np.any(h[m:])
+
+
+def test_cpp_compat_harness():
+    """The reference-named C++ wrappers (include/ggks_compat.hpp) through a
+    g++-built harness that checks them against std::sort on host copies."""
+    import subprocess
+    exe = os.path.join(ROOT, "tests", "cpp", "compat_harness")
+    assert os.path.exists(exe), "built by ggks-multiselect_amd/build.py"
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0 and "ALL OK" in r.stdout, r.stdout + r.stderr
