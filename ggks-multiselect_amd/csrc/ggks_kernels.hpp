@@ -86,12 +86,22 @@ __device__ uint32_t block_scan_array(uint32_t* a, uint32_t L, uint32_t* wsum) {
   const uint32_t seg = ((L + NW - 1) / NW + 63) / 64 * 64;
   const uint32_t b0 = w * seg, b1 = min(L, b0 + seg);
   uint32_t carry = 0;
-  for (uint32_t base = b0; base < b1; base += 64) {
-    const uint32_t i = base + lane;
-    const uint32_t v = i < b1 ? a[i] : 0u;
-    const uint32_t inc = wave_incl_scan(v);
-    if (i < b1) a[i] = carry + inc - v;
-    carry += wave_last(inc);
+  constexpr uint32_t U = 8;  // independent wavefront scans in flight
+  for (uint32_t base = b0; base < b1; base += 64 * U) {
+    uint32_t v[U], inc[U];
+#pragma unroll
+    for (uint32_t u = 0; u < U; ++u) {
+      const uint32_t i = base + u * 64 + lane;
+      v[u] = i < b1 ? a[i] : 0u;
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < U; ++u) inc[u] = wave_incl_scan(v[u]);
+#pragma unroll
+    for (uint32_t u = 0; u < U; ++u) {
+      const uint32_t i = base + u * 64 + lane;
+      if (i < b1) a[i] = carry + inc[u] - v[u];
+      carry += wave_last(inc[u]);
+    }
   }
   if (lane == 0) wsum[w] = carry;
   __syncthreads();
@@ -158,8 +168,6 @@ struct BucketFn {
     rsh = st.rsh;
     twid = st.twid;
     lsh = st.lsh;
-    sc_off = st.sc_off;
-    sc_wid = st.sc_wid;
   }
   // stage 1: LUT index p and the normalised sub-field t of a key
   __device__ __forceinline__ void split(KT key, uint32_t& p, uint32_t& t) const {
@@ -184,16 +192,6 @@ struct BucketFn {
     uint32_t p, t;
     split(key, p, t);
     return finish(lut[p], t);
-  }
-  // candidate cell: the prefix and the top sc_wid bits of the sub-field,
-  // (p << sc_wid) | c -- one bit field of low, as the sub-field's top bits sit
-  // right below the prefix (sc_off + sc_wid = poff); no LUT, no multiply
-  uint32_t sc_off, sc_wid;
-  __device__ __forceinline__ uint32_t cell(KT key) const {
-    KT low = (key > kb ? key : kb) - kb;
-    low = low < lmax ? low : lmax;
-    if (sizeof(KT) == 4) return __builtin_amdgcn_ubfe((uint32_t)low, sc_off, sc_wid + pwid);
-    return (uint32_t)((uint64_t)low >> sc_off) & ((1u << (sc_wid + pwid)) - 1u);
   }
 };
 
@@ -521,8 +519,6 @@ __global__ __launch_bounds__(kThreads) void k_lut(const SegDesc* __restrict__ se
     st.twid = s < 24 ? s : 24;
     st.rsh = s > 24 ? s - 24 : 0;
     st.lsh = s < 24 ? 24 - s : 0;
-    st.sc_wid = s < kCellBits ? s : kCellBits;
-    st.sc_off = s - st.sc_wid;
     states[blockIdx.x] = st;
   }
 }
@@ -588,7 +584,8 @@ __global__ __launch_bounds__(kThreads) void k_count(const KT* __restrict__ src,
 }
 
 // Sum the chunk rows of kReduceBins buckets; rewrite each row entry as the
-// chunk's exclusive offset inside its bucket.  1024 threads = 16 groups of
+// chunk's exclusive offset inside its bucket; write the group's bucket
+// prefixes and its sum.  1024 threads = 16 groups of
 // chunks x 64 bins; each thread keeps its (at most kReduceRegs) row entries in
 // registers between the sum and the rewrite.
 constexpr uint32_t kReduceParts = kThreads / kReduceBins;  // 16
@@ -598,7 +595,8 @@ __global__ __launch_bounds__(kThreads) void k_reduce(const SegDesc* __restrict__
                                                      const SegState* __restrict__ states,
                                                      const ReduceItem* __restrict__ items,
                                                      uint32_t* __restrict__ rows,
-                                                     uint32_t* __restrict__ totals) {
+                                                     uint32_t* __restrict__ totals,
+                                                     uint32_t* __restrict__ gsums) {
   __shared__ uint32_t part[kReduceParts][kReduceBins];
   const ReduceItem it = items[blockIdx.x];
   const SegDesc& sd = segs[it.seg];
@@ -633,7 +631,14 @@ __global__ __launch_bounds__(kThreads) void k_reduce(const SegDesc* __restrict__
     if (valid && c0 + u < c1) base[(uint64_t)(c0 + u) * stride] = run;
     run += v[u];
   }
-  if (valid && q == 0) totals[sd.tot_off + bin] = total;
+  // wavefront 0 holds the 64 bins of the item: their exclusive prefix inside
+  // the group, and the group's sum (k_findk scans the group sums only)
+  if (q == 0) {
+    const uint32_t t = valid ? total : 0u;
+    const uint32_t inc = wave_incl_scan(t);
+    if (valid) totals[sd.tot_off + bin] = inc - t;
+    if (lane == kReduceBins - 1) gsums[sd.tot_off / kReduceBins + it.group] = inc;
+  }
 }
 
 // The i-th query of a segment: explicit, from a raw k list (k-th largest ->
@@ -641,11 +646,12 @@ __global__ __launch_bounds__(kThreads) void k_reduce(const SegDesc* __restrict__
 // to k = 1), or rank len - i for a top-K.
 __device__ __forceinline__ Query seg_query(const SegDesc& sd, const Query* __restrict__ queries,
                                            const uint32_t* __restrict__ kraw, uint32_t i,
-                                           uint32_t* __restrict__ status) {
+                                           uint32_t* __restrict__ status,
+                                           const uint32_t* kpre = nullptr) {
   Query q;
   if (sd.qsrc == QSRC_QUERIES) return queries[sd.q_off + i];
   if (sd.qsrc == QSRC_K) {
-    uint32_t k = kraw[sd.q_off + i];
+    uint32_t k = kpre ? *kpre : kraw[sd.q_off + i];
     if (k < 1 || (uint64_t)k > sd.len) {
       atomicOr(status, (uint32_t)STATUS_BAD_K);
       k = 1;
@@ -658,6 +664,12 @@ __device__ __forceinline__ Query seg_query(const SegDesc& sd, const Query* __res
   return q;
 }
 
+// Output slot of the i-th query of a segment (no k list read).
+__device__ __forceinline__ uint32_t seg_out(const SegDesc& sd, const Query* __restrict__ queries,
+                                            uint32_t i) {
+  return sd.qsrc == QSRC_QUERIES ? queries[sd.q_off + i].out : sd.out_off + i;
+}
+
 // Per segment: ranks -> buckets, children, the child map for the compaction
 // pass, and the ranks grouped by child.
 template <typename KT, int C>
@@ -665,6 +677,7 @@ __global__ __launch_bounds__(kThreads) void k_findk(const SegDesc* __restrict__ 
                                                     SegState* __restrict__ states,
                                                     const uint32_t* __restrict__ lut_pool,
                                                     const uint32_t* __restrict__ totals,
+                                                    const uint32_t* __restrict__ gsums,
                                                     const Query* __restrict__ queries,
                                                     const uint32_t* __restrict__ kraw,
                                                     uint32_t* __restrict__ status,
@@ -674,10 +687,13 @@ __global__ __launch_bounds__(kThreads) void k_findk(const SegDesc* __restrict__ 
                                                     Query* __restrict__ cq,
                                                     uint32_t* __restrict__ bits_pool,
                                                     uint32_t* __restrict__ wbase_pool,
-                                                    uint32_t* __restrict__ cand_pool,
                                                     KT* __restrict__ out) {
   extern __shared__ uint32_t lds[];
   const SegDesc sd = segs[blockIdx.x];
+  // the first k of this thread, read now so its latency hides under the
+  // table loads and scans
+  const uint32_t kfirst =
+      (sd.qsrc == QSRC_K && threadIdx.x < sd.q_cnt) ? kraw[sd.q_off + threadIdx.x] : 0u;
   __shared__ SegState st;
   if (threadIdx.x == 0) st = states[blockIdx.x];
   __syncthreads();
@@ -691,14 +707,16 @@ __global__ __launch_bounds__(kThreads) void k_findk(const SegDesc* __restrict__ 
   uint32_t* wc = wb + wcap;                      // wcap: first reduced slot of a word
   uint32_t* wsum = wc + wcap;                    // 17 (+ pad)
   const uint32_t tid = threadIdx.x;
+  // bucket prefixes inside groups of kReduceBins, and the group sums
+  const uint32_t ngroups = (B + kReduceBins - 1) / kReduceBins;  // <= wcap
   strided_for(totals + sd.tot_off, B, [&](uint32_t i, uint32_t v) { hist[i] = v; });
+  strided_for(gsums + sd.tot_off / kReduceBins, ngroups, [&](uint32_t i, uint32_t v) { wc[i] = v; });
   strided_for(lut_pool + sd.lut_off, st.P, [&](uint32_t i, uint32_t v) { lut[i] = v; });
   for (uint32_t i = tid; i < nwords; i += kThreads) bits[i] = 0;
-  uint32_t* cand = cand_pool + sd.cand_off;
-  const uint32_t cwords = st.P << (kCellBits - 5);
-  for (uint32_t i = tid; i < cwords; i += kThreads) cand[i] = 0;
   __syncthreads();
-  const uint32_t len = block_scan_array(hist, B, wsum);  // hist = exclusive
+  const uint32_t len = block_scan_array(wc, ngroups, wsum);  // group bases
+  for (uint32_t i = tid; i < B; i += kThreads) hist[i] += wc[i / kReduceBins];
+  __syncthreads();  // hist = exclusive prefix of the bucket counts
   RangeOf<KT> range;
   range.fn.init(st);
   range.lut = lut;
@@ -710,15 +728,18 @@ __global__ __launch_bounds__(kThreads) void k_findk(const SegDesc* __restrict__ 
   // A: bucket of every rank; single-key buckets are answered here
   const uint32_t NONE = 0xFFFFFFFFu;
   for (uint32_t i = tid; i < sd.q_cnt; i += kThreads) {
-    const Query q = seg_query(sd, queries, kraw, i, status);
+    const Query q = seg_query(sd, queries, kraw, i, status, i == tid ? &kfirst : nullptr);
     const uint32_t r1 = q.rank - 1;
     uint32_t lo = 0, hi = B - 1;  // largest b with hist[b] <= r1
     while (lo < hi) {
       uint32_t mid = (lo + hi + 1) >> 1;
       if (hist[mid] <= r1) lo = mid; else hi = mid - 1;
     }
-    KT klo, khi;
-    range(lo, klo, khi);
+    // a bucket inside a prefix of 2^s keys split nb < 2^16 ways spans at
+    // least floor(2^s / nb) >= 2 keys when s >= 17 (edge buckets only
+    // widen), so its exact range is needed only for smaller prefixes
+    KT klo = 0, khi = 1;
+    if (st.s < 17) range(lo, klo, khi);
     if (klo == khi) {
       out[q.out] = from_key<C>(klo);
       qb[sd.q_off + i] = NONE;
@@ -776,14 +797,6 @@ __global__ __launch_bounds__(kThreads) void k_findk(const SegDesc* __restrict__ 
     c.bucket = b;
     c.seg = blockIdx.x;
     children[sd.q_off + u] = c;
-    // mark every candidate cell the bucket's keys can fall in
-    const uint32_t c0 = range.fn.cell(klo), c1 = range.fn.cell(khi);
-    for (uint32_t w = c0 >> 5; w <= (c1 >> 5); ++w) {
-      const uint32_t lo_b = w == (c0 >> 5) ? (c0 & 31u) : 0u;
-      const uint32_t hi_b = w == (c1 >> 5) ? (c1 & 31u) : 31u;
-      const uint32_t m = (hi_b == 31u ? 0xFFFFFFFFu : ((2u << hi_b) - 1u)) & ~((1u << lo_b) - 1u);
-      atomicOr(&cand[w], m);
-    }
   }
   __syncthreads();
 
@@ -813,7 +826,7 @@ __global__ __launch_bounds__(kThreads) void k_findk(const SegDesc* __restrict__ 
     uint32_t pos = atomicAdd(&hist[u], 1u);
     Query q;
     q.rank = qr[sd.q_off + i];
-    q.out = seg_query(sd, queries, kraw, i, status).out;
+    q.out = seg_out(sd, queries, i);
     cq[sd.q_off + pos] = q;
   }
   // unused child slots finish nothing
@@ -826,27 +839,23 @@ __global__ __launch_bounds__(kThreads) void k_findk(const SegDesc* __restrict__ 
 
 // Pass 2 (k_compact): stream the chunk once and move the keys of active
 // buckets into their children's ranges (each chunk owns a private slice of
-// every child: its running offsets start at the chunk's row entries).  The
-// first filter per key is its candidate cell (prefix plus the top bits of its
-// sub-field, no LUT look-up and no multiply) tested in an LDS bitmap that
-// marks every cell overlapping an active bucket.  Per round a lane spills its
-// keys to LDS and lists the indices of its candidates (slots from one
-// wavefront scan); the wavefront then settles the listed candidates 64 at a
-// time: exact bucket, active test, LDS cursor, store.
+// every child: its running offsets start at the chunk's row entries).  Every
+// key is bucketed exactly as in pass 1 (LUT in static LDS) and tested in the
+// active-bucket bitmap; per round a lane spills its keys to LDS and lists the
+// indices of its active keys (slots from one wavefront scan); the wavefront
+// then settles the listed keys 64 at a time: bucket, child, LDS cursor,
+// store.
 constexpr uint32_t kWaves = kThreads / 64;
 constexpr int kRound = 4;                   // 16-byte loads per lane per round
 constexpr uint32_t kCompactCursors = 1024;  // LDS cursors; children beyond use global atomics
 
-// candidate-cell bitmap of the largest prefix table, in static LDS at a
-// fixed address (the word address of a cell then needs no base add)
-constexpr uint32_t kCandWords = (1u << kMaxPrefixBits) << (kCellBits - 5);
-
 template <typename KT>
 __host__ __device__ constexpr size_t compact_lds_bytes(uint32_t pbits, uint32_t bmax) {
-  // dynamic part: LUT, (bits, child base) per word, cursors, the round's keys
-  // of every lane (64 B each) and the staged candidate indices
-  return (size_t)(1u << pbits) * 4 + (size_t)((bmax + 31) / 32) * 8 +
-         (size_t)kCompactCursors * 4 + (size_t)kThreads * kRound * 16 +
+  // dynamic part: cursors, the round's keys of every lane (64 B each) and
+  // the staged key indices (the LUT and bitmaps are static)
+  (void)pbits;
+  (void)bmax;
+  return (size_t)kCompactCursors * 4 + (size_t)kThreads * kRound * 16 +
          (size_t)kThreads * kRound * (16 / sizeof(KT)) * 2;
 }
 
@@ -859,11 +868,13 @@ __global__ __launch_bounds__(kThreads) void k_compact(const KT* __restrict__ src
                                                       uint32_t* __restrict__ rows,
                                                       const uint32_t* __restrict__ bits_pool,
                                                       const uint32_t* __restrict__ wbase_pool,
-                                                      const uint32_t* __restrict__ cand_pool,
                                                       const Child* __restrict__ children,
                                                       KT* __restrict__ red) {
   extern __shared__ __align__(16) uint32_t lds[];
-  __shared__ uint32_t cbm[kCandWords];
+  // static LDS at fixed addresses: no base add in the per-key address math
+  __shared__ uint32_t lut[1u << kMaxPrefixBits];
+  __shared__ uint32_t abits[kMaxBuckets / 32];  // active buckets
+  __shared__ uint32_t wbase[kMaxBuckets / 32];  // first child of each bitmap word
   const Chunk ch = chunks[blockIdx.x];
   const SegDesc& sd = segs[ch.seg];
   const SegState st = states[ch.seg];
@@ -872,18 +883,15 @@ __global__ __launch_bounds__(kThreads) void k_compact(const KT* __restrict__ src
   constexpr uint32_t VEC = 16 / sizeof(KT);
   constexpr int N = kRound * VEC;  // keys per lane per round
   const uint32_t nwords = (st.B + 31) / 32;
-  const uint32_t cwords = st.P << (kCellBits - 5);
-  char* base = reinterpret_cast<char*>(lds);
-  uint32_t* lut = reinterpret_cast<uint32_t*>(base);
-  uint2* bw = reinterpret_cast<uint2*>(lut + (1u << sd.pbits));  // (bits, child base)
-  uint32_t* cur = reinterpret_cast<uint32_t*>(bw + (sd.bmax + 31) / 32);
+  uint32_t* cur = lds;
   uint4* kbuf = reinterpret_cast<uint4*>(cur + kCompactCursors);  // kRound per lane
   uint16_t* sidx = reinterpret_cast<uint16_t*>(kbuf + kThreads * kRound);
   const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = lane_id();
-  strided_for(cand_pool + sd.cand_off, cwords, [&](uint32_t i, uint32_t v) { cbm[i] = v; });
   strided_for(lut_pool + sd.lut_off, st.P, [&](uint32_t i, uint32_t v) { lut[i] = v; });
-  for (uint32_t i = tid; i < nwords; i += kThreads)
-    bw[i] = make_uint2(bits_pool[sd.bits_off + i], wbase_pool[sd.bits_off + i]);
+  for (uint32_t i = tid; i < nwords; i += kThreads) {
+    abits[i] = bits_pool[sd.bits_off + i];
+    wbase[i] = wbase_pool[sd.bits_off + i];
+  }
   uint32_t* row = rows + sd.row_base + (uint64_t)ch.idx * sd.bmax;
   const Child* kids = children + sd.q_off;
   for (uint32_t u = tid; u < U && u < kCompactCursors; u += kThreads)
@@ -891,13 +899,13 @@ __global__ __launch_bounds__(kThreads) void k_compact(const KT* __restrict__ src
   __syncthreads();
   BucketFn<KT> bk;
   bk.init(st);
-  // exact test of a candidate; stores it into its child's range if active
+  // stores a key into its child's range if its bucket is active
   auto settle = [&](KT raw) {
     const uint32_t b = bk(to_key<C>(raw), lut);
-    const uint2 e = bw[b >> 5];
+    const uint32_t bits = abits[b >> 5];
     const uint32_t sh = b & 31u;
-    if ((e.x >> sh) & 1u) {
-      const uint32_t u = e.y + __popc(e.x & ((1u << sh) - 1u));
+    if ((bits >> sh) & 1u) {
+      const uint32_t u = wbase[b >> 5] + __popc(bits & ((1u << sh) - 1u));
       const uint32_t pos = u < kCompactCursors
                                ? atomicAdd(&cur[u], 1u)
                                : (uint32_t)kids[u].start + atomicAdd(&row[kids[u].bucket], 1u);
@@ -926,25 +934,32 @@ __global__ __launch_bounds__(kThreads) void k_compact(const KT* __restrict__ src
   // candidates of round r settle
   uint4 v[kRound];
   uint32_t okmask = 0;
+  const uint64_t full_rounds = nvec / per_round;  // rounds with every load in range
   auto load_round = [&](uint64_t r) {
+    const uint4* q = pv + r * per_round + tid;
+    if (r < full_rounds) {  // uniform
+      okmask = (N >= 32) ? 0xFFFFFFFFu : ((1u << N) - 1u);
+#pragma unroll
+      for (int t = 0; t < kRound; ++t) v[t] = stream_load(q + (uint64_t)t * kThreads);
+      return;
+    }
     okmask = 0;
 #pragma unroll
     for (int t = 0; t < kRound; ++t) {
       const uint64_t idx = r * per_round + (uint64_t)t * kThreads + tid;
       const bool ok = idx < nvec;
       okmask |= ok ? (((1u << VEC) - 1u) << (t * VEC)) : 0u;
-      v[t] = ok ? stream_load(&pv[idx]) : make_uint4(0, 0, 0, 0);
+      v[t] = ok ? stream_load(q + (uint64_t)t * kThreads) : make_uint4(0, 0, 0, 0);
     }
   };
   if (rounds) load_round(0);
   for (uint64_t r = 0; r < rounds; ++r) {
     KT k[N];
     unpack<KT, kRound>(v, k);
-    uint32_t c[N], mask = 0;
+    uint32_t b[N], mask = 0;
+    bucket_batch<KT, C, N>(bk, lut, k, b);
 #pragma unroll
-    for (int i = 0; i < N; ++i) c[i] = bk.cell(to_key<C>(k[i]));
-#pragma unroll
-    for (int i = 0; i < N; ++i) mask |= bitmap_test(cbm, c[i]) << i;
+    for (int i = 0; i < N; ++i) mask |= bitmap_test(abits, b[i]) << i;
     mask &= okmask;
     const uint32_t cl = __popc(mask);
     const uint32_t inc = wave_incl_scan(cl);
@@ -981,24 +996,30 @@ __global__ __launch_bounds__(kThreads) void k_copy16(const uint4* __restrict__ s
     dst[i] = stream_load(&src[i]);
 }
 
-// Children with few ranks are finished by one wavefront each (k_finish_wave);
-// the others by a whole work-group (k_finish).
+// Children with few ranks are finished by k_finish_small (one 256-thread
+// work-group each); the others by k_finish (a 1024-thread work-group with
+// the child in LDS).
 constexpr uint32_t kWaveRanks = 4;
 constexpr uint64_t kWaveMaxLen = 1ull << 15;
 __device__ __forceinline__ bool wave_child(const Child& c) {
   return c.q_cnt <= kWaveRanks && c.len <= kWaveMaxLen;
 }
+constexpr uint32_t kFThreads = 256;
+constexpr uint32_t kFDigit = 11;                      // first digit: 2048 bins
+constexpr uint32_t kFWords = (1u << kFDigit) + (1u << kFDigit) / 32;  // padded: bin b at b + b/32
+constexpr uint32_t kFCollectWords = 1024;             // keys gathered after the first digit
 
-// Visit keys p[0, len) of one child with a wavefront: 16-byte loads for the
-// aligned body (eight in flight per lane), scalar head and tail.
+// Visit keys p[0, len) with a work-group of kFThreads: 16-byte loads for the
+// aligned body (eight in flight per thread), scalar head and tail.
 template <typename KT, typename F>
-__device__ __forceinline__ void wave_for_each(const KT* __restrict__ p, uint64_t len, uint32_t lane,
-                                              F&& f) {
+__device__ __forceinline__ void group_for_each(const KT* __restrict__ p, uint64_t len, F&& f) {
   constexpr uint32_t VEC = 16 / sizeof(KT);
+  constexpr int D = 8;
+  const uint32_t tid = threadIdx.x;
   uint64_t mis = ((uintptr_t)p & 15u) / sizeof(KT);
   uint64_t head = mis ? (VEC - mis) : 0;
   if (head > len) head = len;
-  if (lane < head) f(p[lane]);
+  if (tid < head) f(p[tid]);
   const uint64_t nvec = (len - head) / VEC;
   const uint4* __restrict__ pv = reinterpret_cast<const uint4*>(p + head);
   auto apply = [&](const uint4& v) {
@@ -1009,50 +1030,77 @@ __device__ __forceinline__ void wave_for_each(const KT* __restrict__ p, uint64_t
       f((KT)(((uint64_t)v.w << 32) | v.z));
     }
   };
-  uint64_t i = lane;
-  for (; i + 7 * 64 < nvec; i += 8 * 64) {
-    uint4 v[8];
+  uint64_t i = tid;
+  for (; i + (D - 1) * kFThreads < nvec; i += D * kFThreads) {
+    uint4 v[D];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) v[u] = pv[i + u * 64];
+    for (int u = 0; u < D; ++u) v[u] = pv[i + u * kFThreads];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) apply(v[u]);
+    for (int u = 0; u < D; ++u) apply(v[u]);
   }
-  for (; i < nvec; i += 64) apply(pv[i]);
+  for (; i < nvec; i += kFThreads) apply(pv[i]);
   const uint64_t done = head + nvec * VEC;
-  if (done + lane < len) f(p[done + lane]);
+  if (done + tid < len) f(p[done + tid]);
 }
 
-// Finisher for children with at most kWaveRanks ranks: one wavefront per
-// child, MSD radix select on 8-bit digits of (key - lo) with a private
-// 256-bin LDS histogram per wavefront; the keys are re-read (from L2) each
-// digit pass instead of being staged.  No work-group barriers, so every
-// child of a launch runs at once.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Rank r (0-based) among 256 LDS bins h[0, 2^d) counted by a wavefront: the
+// bin holding it and the number of values in lower bins.
+__device__ __forceinline__ void wave_pick256(const uint32_t* h, uint32_t lane, uint32_t r,
+                                             uint32_t& bin, uint32_t& below) {
+  const uint32_t v0 = h[lane * 4], v1 = h[lane * 4 + 1], v2 = h[lane * 4 + 2], v3 = h[lane * 4 + 3];
+  const uint32_t sum = v0 + v1 + v2 + v3;
+  const uint32_t e0 = wave_incl_scan(sum) - sum;
+  const uint32_t e1 = e0 + v0, e2 = e1 + v1, e3 = e2 + v2;
+  int hit = -1;
+  uint32_t start = 0;
+  if (r >= e0 && r < e1) { hit = 0; start = e0; }
+  if (r >= e1 && r < e2) { hit = 1; start = e1; }
+  if (r >= e2 && r < e3) { hit = 2; start = e2; }
+  if (r >= e3 && r < e3 + v3) { hit = 3; start = e3; }
+  const int src = __ffsll((unsigned long long)__ballot(hit >= 0)) - 1;
+  bin = (uint32_t)__shfl((int)(lane * 4 + hit), src, 64);
+  below = (uint32_t)__shfl((int)start, src, 64);
+}
+
+// Finisher for children with at most kWaveRanks ranks: one 256-thread
+// work-group per child (several per CU), two passes over the child's keys in
+// memory.  Pass 1 builds an 11-bit top-digit histogram of (key - lo) in a
+// padded LDS table (lane l's 32 bins are bank disjoint); wavefront 0 then
+// finds each rank's top digit.  Pass 2 gathers, per rank, the keys holding
+// that digit (a handful, as the child is one bucket) into LDS, where
+// wavefront 0 resolves the rank: by counting for up to 64 keys, by 8-bit
+// radix passes over LDS up to the gather capacity, and beyond it (heavy
+// duplicates of one digit) by 8-bit radix passes over the child in memory.
 template <typename KT, int C>
-__global__ __launch_bounds__(kThreads) void k_finish_wave(const KT* __restrict__ input,
-                                                          const KT* __restrict__ red,
-                                                          const Child* __restrict__ children,
-                                                          uint32_t nchildren,
-                                                          const Query* __restrict__ cq,
-                                                          KT* __restrict__ out) {
+__global__ __launch_bounds__(kFThreads) void k_finish_small(const KT* __restrict__ input,
+                                                            const KT* __restrict__ red,
+                                                            const Child* __restrict__ children,
+                                                            uint32_t nchildren,
+                                                            const Query* __restrict__ cq,
+                                                            KT* __restrict__ out) {
   constexpr uint32_t W = KeyBits<KT>::W;
-  __shared__ uint32_t hist_all[kThreads / 64][256];
-  const uint32_t lane = lane_id();
-  const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-  uint32_t* h = hist_all[wave];
-  auto wave_sync = [] {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  };
-  for (uint32_t ci = blockIdx.x * (kThreads / 64) + wave; ci < nchildren;
-       ci += gridDim.x * (kThreads / 64)) {
+  constexpr uint32_t CAP = kFCollectWords * 4 / sizeof(KT);  // gathered keys
+  __shared__ uint32_t h[kFWords];
+  __shared__ KT red_lo[kFThreads / 64], red_hi[kFThreads / 64];
+  __shared__ uint32_t res[4 * kWaveRanks + 2];  // per rank: digit, residual, count; gathered
+  const uint32_t tid = threadIdx.x, lane = lane_id();
+  const bool w0 = tid < 64;
+  KT* buf = reinterpret_cast<KT*>(h);  // gathered keys (after the histogram is read)
+  uint32_t* bins = h + kFCollectWords;  // 256 bins for the LDS radix passes
+  for (uint32_t ci = blockIdx.x; ci < nchildren; ci += gridDim.x) {
     const Child c = children[ci];
-    if (c.q_cnt == 0 || !wave_child(c)) continue;
+    if (c.q_cnt == 0 || !wave_child(c)) continue;  // uniform
     const KT* srcp = (c.src ? red : input) + c.start;
     KT lo = (KT)c.lo, hi = (KT)c.hi;
     if (c.need_range) {
       KT a = KeyBits<KT>::MAX, b = 0;
-      wave_for_each(srcp, c.len, lane, [&](KT raw) {
+      group_for_each(srcp, c.len, [&](KT raw) {
         const KT k = to_key<C>(raw);
         a = k < a ? k : a;
         b = k > b ? k : b;
@@ -1062,52 +1110,146 @@ __global__ __launch_bounds__(kThreads) void k_finish_wave(const KT* __restrict__
         a = x < a ? x : a;
         b = y > b ? y : b;
       }
-      lo = a;
-      hi = b;
+      if (lane == 0) {
+        red_lo[tid >> 6] = a;
+        red_hi[tid >> 6] = b;
+      }
+      __syncthreads();
+      lo = red_lo[0];
+      hi = red_hi[0];
+      for (uint32_t w = 1; w < kFThreads / 64; ++w) {
+        lo = red_lo[w] < lo ? red_lo[w] : lo;
+        hi = red_hi[w] > hi ? red_hi[w] : hi;
+      }
     }
     const int nbits = bitlen<KT>((KT)(hi - lo));
+    if (nbits == 0) {  // every key equal
+      for (uint32_t qi = tid; qi < c.q_cnt; qi += kFThreads)
+        out[cq[c.q_off + qi].out] = from_key<C>(lo);
+      __syncthreads();
+      continue;
+    }
+    // pass 1: histogram of the top digit
+    const int d1 = nbits < (int)kFDigit ? nbits : (int)kFDigit;
+    const int sh1 = nbits - d1;
+    for (uint32_t j = tid; j < kFWords; j += kFThreads) h[j] = 0;
+    __syncthreads();
+    group_for_each(srcp, c.len, [&](KT raw) {
+      const uint32_t bn = (uint32_t)((KT)(to_key<C>(raw) - lo) >> sh1);
+      atomicAdd(&h[bn + (bn >> 5)], 1u);
+    });
+    __syncthreads();
+    if (w0) {  // each rank's top digit, residual rank and digit count
+      uint32_t lsum = 0;  // lane's bins lane*32 .. lane*32 + 31
+#pragma unroll 8
+      for (uint32_t j = 0; j < 32; ++j) lsum += h[lane * 33 + j];
+      const uint32_t lex = wave_incl_scan(lsum) - lsum;
+      for (uint32_t qi = 0; qi < c.q_cnt; ++qi) {
+        const uint32_t r = cq[c.q_off + qi].rank - 1;
+        const int L = __ffsll((unsigned long long)__ballot(r >= lex && r < lex + lsum)) - 1;
+        const uint32_t eL = (uint32_t)__shfl((int)lex, L, 64);
+        const uint32_t v = lane < 32 ? h[(uint32_t)L * 33 + lane] : 0u;
+        const uint32_t ex = wave_incl_scan(v) - v + eL;
+        const int j = __ffsll((unsigned long long)__ballot(lane < 32 && r >= ex && r < ex + v)) - 1;
+        const uint32_t exj = (uint32_t)__shfl((int)ex, j, 64);  // every lane takes part
+        const uint32_t vj = (uint32_t)__shfl((int)v, j, 64);
+        if (lane == 0) {
+          res[3 * qi] = (uint32_t)L * 32 + (uint32_t)j;
+          res[3 * qi + 1] = r - exj;
+          res[3 * qi + 2] = vj;
+        }
+      }
+    }
+    __syncthreads();
+    const KT lowmask = sh1 == 0 ? KT(0) : (KT)(KeyBits<KT>::MAX >> (W - sh1));
     for (uint32_t qi = 0; qi < c.q_cnt; ++qi) {
       const Query q = cq[c.q_off + qi];
-      uint32_t r1 = q.rank - 1;  // rank among the keys with the current prefix
-      KT pre = 0;
-      int rem = nbits;
-      while (rem > 0) {
-        const int d = rem < 8 ? rem : 8;
-        const int shift = rem - d;
-        const int hshift = shift + d;
-        const uint32_t dmask = (1u << d) - 1u;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) h[lane * 4 + j] = 0;
-        wave_sync();
-        wave_for_each(srcp, c.len, lane, [&](KT raw) {
+      const KT top = (KT)res[3 * qi];
+      uint32_t r = res[3 * qi + 1];
+      const uint32_t n2 = res[3 * qi + 2];
+      KT low = 0;  // the rank's key minus lo, below the top digit
+      if (sh1 > 0 && n2 <= CAP) {
+        // pass 2: gather the keys with this top digit
+        if (tid == 0) res[4 * kWaveRanks] = 0;
+        __syncthreads();
+        group_for_each(srcp, c.len, [&](KT raw) {
           const KT rel = (KT)(to_key<C>(raw) - lo);
-          const KT top = hshift >= (int)W ? KT(0) : (KT)(rel >> hshift);
-          if (top == pre) atomicAdd(&h[(uint32_t)(rel >> shift) & dmask], 1u);
+          if ((KT)(rel >> sh1) == top) buf[atomicAdd(&res[4 * kWaveRanks], 1u)] = rel & lowmask;
         });
-        wave_sync();
-        // bins 4 lane .. 4 lane + 3; exclusive starts across the wavefront
-        const uint32_t v0 = h[lane * 4], v1 = h[lane * 4 + 1], v2 = h[lane * 4 + 2],
-                       v3 = h[lane * 4 + 3];
-        const uint32_t sum = v0 + v1 + v2 + v3;
-        const uint32_t e0 = wave_incl_scan(sum) - sum;
-        const uint32_t e1 = e0 + v0, e2 = e1 + v1, e3 = e2 + v2;
-        // the bin holding rank r1: start <= r1 < start + count (exactly one)
-        int hit = -1;
-        uint32_t start = 0;
-        if (r1 >= e0 && r1 < e1) { hit = 0; start = e0; }
-        if (r1 >= e1 && r1 < e2) { hit = 1; start = e1; }
-        if (r1 >= e2 && r1 < e3) { hit = 2; start = e2; }
-        if (r1 >= e3 && r1 < e3 + v3) { hit = 3; start = e3; }
-        const uint64_t who = __ballot(hit >= 0);
-        const int src = __ffsll((unsigned long long)who) - 1;
-        const uint32_t dg = (uint32_t)__shfl(lane * 4 + hit, src, 64);
-        r1 -= (uint32_t)__shfl(start, src, 64);
-        pre = (KT)((pre << d) | (KT)dg);
-        rem = shift;
-        wave_sync();
+        __syncthreads();
+        if (w0) {
+          if (n2 <= 64) {
+            // rank by counting: lane i holds key i; ties broken by position
+            const KT mine = lane < n2 ? buf[lane] : KeyBits<KT>::MAX;
+            uint32_t below = 0;
+            for (uint32_t j = 0; j < n2; ++j) {
+              const KT o = buf[j];
+              below += (o < mine || (o == mine && j < lane)) ? 1u : 0u;
+            }
+            const int src = __ffsll((unsigned long long)__ballot(lane < n2 && below == r)) - 1;
+            low = (KT)__shfl(mine, src, 64);
+          } else {
+            int rem = sh1;
+            KT pre = 0;
+            while (rem > 0) {
+              const int d = rem < 8 ? rem : 8;
+              const int shift = rem - d;
+              const int hshift = shift + d;
+#pragma unroll
+              for (int j = 0; j < 4; ++j) bins[lane * 4 + j] = 0;
+              wave_sync();
+              for (uint32_t i = lane; i < n2; i += 64) {
+                const KT v = buf[i];
+                const KT t = hshift >= (int)W ? KT(0) : (KT)(v >> hshift);
+                if (t == pre) atomicAdd(&bins[(uint32_t)(v >> shift) & ((1u << d) - 1u)], 1u);
+              }
+              wave_sync();
+              uint32_t bn, below;
+              wave_pick256(bins, lane, r, bn, below);
+              r -= below;
+              pre = (KT)((pre << d) | (KT)bn);
+              rem = shift;
+              wave_sync();
+            }
+            low = pre;
+          }
+        }
+        __syncthreads();
+      } else if (sh1 > 0) {
+        // many keys share the top digit: 8-bit passes over the child in memory
+        int rem = sh1;
+        KT pre = top;
+        while (rem > 0) {
+          const int d = rem < 8 ? rem : 8;
+          const int shift = rem - d;
+          const int hshift = shift + d;
+          if (tid < 256) bins[tid] = 0;
+          __syncthreads();
+          group_for_each(srcp, c.len, [&](KT raw) {
+            const KT rel = (KT)(to_key<C>(raw) - lo);
+            if ((KT)(rel >> hshift) == pre)
+              atomicAdd(&bins[(uint32_t)(rel >> shift) & ((1u << d) - 1u)], 1u);
+          });
+          __syncthreads();
+          if (w0) {
+            uint32_t bn, below;
+            wave_pick256(bins, lane, r, bn, below);
+            if (lane == 0) {
+              res[4 * kWaveRanks + 1] = bn;
+              res[4 * kWaveRanks] = below;
+            }
+          }
+          __syncthreads();
+          r -= res[4 * kWaveRanks];
+          pre = (KT)((pre << d) | (KT)res[4 * kWaveRanks + 1]);
+          rem = shift;
+          __syncthreads();
+        }
+        low = (KT)(pre & lowmask);
       }
-      if (lane == 0) out[q.out] = from_key<C>((KT)(lo + pre));
+      if (tid == 0) out[q.out] = from_key<C>((KT)(lo + ((KT)(top << sh1) | low)));
     }
+    __syncthreads();
   }
 }
 

@@ -48,7 +48,8 @@ struct Plan {
   // arena offsets (bytes)
   size_t off_tables = 0, off_segs = 0, off_chunks = 0, off_items = 0, tables_end = 0;
   size_t off_states = 0, off_samples = 0, off_minmax = 0, off_lut = 0, off_rows = 0, off_tot = 0;
-  size_t off_bits = 0, off_wbase = 0, off_cand = 0, off_q = 0, off_qb = 0, off_qr = 0;
+  size_t off_bits = 0, off_wbase = 0, off_q = 0, off_qb = 0, off_qr = 0;
+  size_t off_gsum = 0;  // per-group bucket sums (k_reduce -> k_findk)
   size_t off_children = 0, off_cq = 0, off_red = 0, total = 0;
   size_t off_kvals = 0, off_dchildren = 0, off_dqueries = 0;  // batch / direct tables
   std::vector<uint32_t> kvals;         // k lists of the engine segments, by q_off
@@ -143,7 +144,6 @@ inline void plan_single(Plan& p, int kind, int dtype, uint64_t n, uint32_t nq,
   s.lut_off = 0;
   s.tot_off = 0;
   s.bits_off = 0;
-  s.cand_off = 0;
   s.q_off = 0;
   s.q_cnt = nq;
   s.mode = mode;
@@ -176,9 +176,9 @@ inline void plan_single(Plan& p, int kind, int dtype, uint64_t n, uint32_t nq,
   p.off_lut = take((size_t)(1u << pb) * 4);
   p.off_rows = take((size_t)G * bmax * 4);
   p.off_tot = take((size_t)bmax * 4);
+  p.off_gsum = take((size_t)(bmax / kReduceBins + 1) * 4);
   p.off_bits = take((size_t)((bmax + 31) / 32) * 4);
   p.off_wbase = take((size_t)((bmax + 31) / 32) * 4);
-  p.off_cand = take((size_t)(1u << pb) << (kCellBits - 3));
   p.off_q = take((size_t)nq * sizeof(Query));
   p.off_qb = take((size_t)nq * 4);
   p.off_qr = take((size_t)nq * 4);
@@ -255,7 +255,7 @@ inline void plan_batch(Plan& p, int dtype, const std::vector<BatchSeg>& in, cons
   const TuningTable& tab = tuning_table();
   p.seed = o.seed ? o.seed : 0x6A09E667F3BCC909ull;
   uint64_t red = 0, rowc = 0;
-  uint32_t qoff = 0, sam = 0, lutc = 0, totc = 0, bitc = 0, candc = 0, dq = 0;
+  uint32_t qoff = 0, sam = 0, lutc = 0, totc = 0, bitc = 0, dq = 0;
   for (size_t i = 0; i < in.size(); ++i) {
     const BatchSeg& g = in[i];
     if (g.num_k == 0) continue;
@@ -318,7 +318,6 @@ inline void plan_batch(Plan& p, int dtype, const std::vector<BatchSeg>& in, cons
     s.lut_off = lutc;
     s.tot_off = totc;
     s.bits_off = bitc;
-    s.cand_off = candc;
     s.q_off = qoff;
     s.q_cnt = g.num_k;
     s.mode = mode;
@@ -350,7 +349,6 @@ inline void plan_batch(Plan& p, int dtype, const std::vector<BatchSeg>& in, cons
     lutc += 1u << pb;
     totc += bmax;
     bitc += (bmax + 31) / 32;
-    candc += (1u << pb) << (kCellBits - 5);
   }
   p.nq = qoff;
   size_t off = 0;
@@ -374,9 +372,9 @@ inline void plan_batch(Plan& p, int dtype, const std::vector<BatchSeg>& in, cons
   p.off_lut = take((size_t)lutc * 4);
   p.off_rows = take((size_t)rowc * 4);
   p.off_tot = take((size_t)totc * 4);
+  p.off_gsum = take((size_t)(totc / kReduceBins + 1) * 4);
   p.off_bits = take((size_t)bitc * 4);
   p.off_wbase = take((size_t)bitc * 4);
-  p.off_cand = take((size_t)candc * 4);
   p.off_q = off;  // engine segments read k lists or top-K ranks, no Query records
   p.off_qb = take((size_t)qoff * 4);
   p.off_qr = take((size_t)qoff * 4);

@@ -175,14 +175,15 @@ static void level_attrs(const Plan&) {
   (void)once;
 }
 
-// Both finishers over a child list: one wavefront per child with few ranks,
-// one work-group per child otherwise (each kernel skips the other's).
+// Both finishers over a child list: a 256-thread work-group per child with
+// few ranks, a 1024-thread work-group per child otherwise (each kernel skips the other's).  The
+// work-group finisher's LDS allows one group per CU, so a grid of max_groups
+// (256: one per CU) loops over the list without losing concurrency.
 template <typename KT, int C>
 static void finish(hipStream_t s, const KT* d_src, const KT* red, const Child* kids, uint32_t n,
                    const Query* cq, KT* d_out, uint32_t max_groups) {
-  const uint32_t wgrid = std::min<uint32_t>((n + kThreads / 64 - 1) / (kThreads / 64), 8192);
-  hipLaunchKernelGGL((k_finish_wave<KT, C>), dim3(wgrid), dim3(kThreads), 0, s, d_src, red, kids,
-                     n, cq, d_out);
+  hipLaunchKernelGGL((k_finish_small<KT, C>), dim3(std::min<uint32_t>(n, 65536)), dim3(kFThreads),
+                     0, s, d_src, red, kids, n, cq, d_out);
   hipLaunchKernelGGL((k_finish<KT, C>), dim3(std::min<uint32_t>(n, max_groups)), dim3(kThreads),
                      kFinishLdsTotal, s, d_src, red, kids, n, cq, d_out);
 }
@@ -205,9 +206,9 @@ static int enqueue_level(const Exec& x, const Plan& p, const KT* d_src, KT* d_ou
     uint32_t* lut = reinterpret_cast<uint32_t*>(A + p.off_lut);
     uint32_t* rows = reinterpret_cast<uint32_t*>(A + p.off_rows);
     uint32_t* tot = reinterpret_cast<uint32_t*>(A + p.off_tot);
+    uint32_t* gsum = reinterpret_cast<uint32_t*>(A + p.off_gsum);
     uint32_t* bits = reinterpret_cast<uint32_t*>(A + p.off_bits);
     uint32_t* wbase = reinterpret_cast<uint32_t*>(A + p.off_wbase);
-    uint32_t* cand = reinterpret_cast<uint32_t*>(A + p.off_cand);
     const Query* q = reinterpret_cast<const Query*>(A + p.off_q);
     uint32_t* qb = reinterpret_cast<uint32_t*>(A + p.off_qb);
     uint32_t* qr = reinterpret_cast<uint32_t*>(A + p.off_qr);
@@ -231,19 +232,19 @@ static int enqueue_level(const Exec& x, const Plan& p, const KT* d_src, KT* d_ou
                        chunks, lut, rows);
     stage(x, 4);
     hipLaunchKernelGGL(k_reduce, dim3((uint32_t)p.items.size()), dim3(kThreads), 0, s, segs,
-                       states, items, rows, tot);
+                       states, items, rows, tot, gsum);
     stage(x, 5);
     hipLaunchKernelGGL((k_findk<KT, C>), dim3(nseg), dim3(kThreads),
                        ((size_t)p.bmax_max + (1u << p.pbits_max) + 3 * ((p.bmax_max + 31) / 32) +
                         32) * 4,
-                       s, segs, states, lut, tot, q, d_k, x.status, qb, qr, kids, cq, bits, wbase,
-                       cand, d_out);
+                       s, segs, states, lut, tot, gsum, q, d_k, x.status, qb, qr, kids, cq, bits, wbase,
+                       d_out);
     stage(x, 6);
     hipLaunchKernelGGL((k_compact<KT, C>), dim3(nchunks), dim3(kThreads),
                        compact_lds_bytes<KT>(p.pbits_max, p.bmax_max), s, d_src, segs, states,
-                       chunks, lut, rows, bits, wbase, cand, kids, red);
+                       chunks, lut, rows, bits, wbase, kids, red);
     stage(x, 7);
-    finish<KT, C>(s, d_src, red, kids, p.nq, cq, d_out, 2048);
+    finish<KT, C>(s, d_src, red, kids, p.nq, cq, d_out, 256);
   } else {
     for (int st = 1; st < GGKS_NSTAGES; ++st) stage(x, st);
   }
@@ -251,7 +252,7 @@ static int enqueue_level(const Exec& x, const Plan& p, const KT* d_src, KT* d_ou
   if (ndirect > 0) {
     const Child* dk = reinterpret_cast<const Child*>(A + p.off_dchildren);
     const Query* dq = reinterpret_cast<const Query*>(A + p.off_dqueries);
-    finish<KT, C>(s, d_src, (const KT*)nullptr, dk, ndirect, dq, d_out, 4096);
+    finish<KT, C>(s, d_src, (const KT*)nullptr, dk, ndirect, dq, d_out, 256);
   }
   HIP_TRY(hipGetLastError());
   return GGKS_OK;

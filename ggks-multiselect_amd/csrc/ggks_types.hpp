@@ -15,7 +15,6 @@ constexpr int kThreads = 1024;             // work-group size of every kernel
 constexpr uint32_t kMaxBuckets = 32768;    // histogram bins per segment (LDS)
 constexpr uint32_t kMaxPrefixBits = 12;    // LUT prefixes per segment <= 4096
 constexpr uint32_t kCursorCap = 4096;      // LDS write cursors in compaction
-constexpr uint32_t kCellBits = 6;          // candidate cells per prefix: 2^kCellBits
 constexpr uint32_t kFinishLdsBytes = 65536;// keys kept in LDS by the finisher
 constexpr uint32_t kQCap = 1024;           // queries per finisher batch
 constexpr uint32_t kGroupMax = 32;         // prefix groups per histogram pass
@@ -58,7 +57,6 @@ struct SegDesc {          // planned on the host, read-only on the device
   uint32_t lut_off;       // first LUT entry (u32)
   uint32_t tot_off;       // first total (u32), bmax entries
   uint32_t bits_off;      // first bitmap word (u32), bmax/32 words
-  uint32_t cand_off;      // first candidate-cell bitmap word (u32), 2^(pbits+1) words
   uint32_t q_off, q_cnt;  // its queries; child slots use the same range
   uint32_t mode;          // SegMode
   uint32_t radix_bits;    // MODE_RADIX digit width
@@ -73,8 +71,6 @@ struct SegDesc {          // planned on the host, read-only on the device
 //   t   = ((low >> rsh) & (2^twid - 1)) << lsh   the low s bits of low,
 //                                           normalised to 24 bits
 //   b   = base[p] + (t * nb[p]) >> 24       (one 24-bit multiply-high)
-// and the candidate cell of a key is (p << sc_wid) | (top sc_wid <= kCellBits
-// bits of the sub-field), monotone in the key like the bucket.
 struct SegState {         // written on the device by the LUT kernel
   uint64_t kb;            // base key
   uint64_t kmax;          // largest sampled key (information)
@@ -85,7 +81,6 @@ struct SegState {         // written on the device by the LUT kernel
   uint32_t nact;          // active buckets (children)
   uint32_t poff, pwid;    // prefix bit field
   uint32_t rsh, twid, lsh;// sub-bucket bit field and normalising shift
-  uint32_t sc_off, sc_wid;// candidate cell: top <= 6 bits of the sub-field
   uint32_t pad;
 };
 
