@@ -35,7 +35,7 @@ EXPORTS = [
     "ggks_server_run_trace", "ggks_server_stats_get", "ggks_server_destroy",
 ]
 BATCH_GRAPH = 1
-GEN_UNIFORM, GEN_NORMAL, GEN_LEVELS = 0, 1, 2
+GEN_UNIFORM, GEN_NORMAL, GEN_LEVELS, GEN_CAUCHY, GEN_PLATEAU = 0, 1, 2, 3, 4
 NSTAGES = 8
 STAGES = ["queries", "sample", "lut", "count", "reduce", "findk", "compact", "finish"]
 

@@ -309,10 +309,12 @@ def generate(dtype, family: str, seed: int, n: int, start: int = 0, lo: int = 0,
              out: DeviceArray | None = None, offset: int = 0):
     """Fill a device array (new, or `out` from element `offset`) with n seeded
     values: family "uniform" (floats in [0,1), integers in [lo, hi]),
-    "normal", or "levels" (floats holding the integers lo..hi)."""
+    "normal", "levels" (floats holding the integers lo..hi), "cauchy"
+    (clipped to finite) or "plateau" (0.5 with probability 1/2, else
+    uniform)."""
     d = out if out is not None else DeviceArray(n, dtype)
-    fam = {"uniform": _lib.GEN_UNIFORM, "normal": _lib.GEN_NORMAL,
-           "levels": _lib.GEN_LEVELS}[family]
+    fam = {"uniform": _lib.GEN_UNIFORM, "normal": _lib.GEN_NORMAL, "levels": _lib.GEN_LEVELS,
+           "cauchy": _lib.GEN_CAUCHY, "plateau": _lib.GEN_PLATEAU}[family]
     if offset < 0 or offset + n > d.n:
         raise ValueError("generate range outside the array")
     check(lib().ggks_generate(d.ptr + offset * d.dtype.itemsize, dtype_code(d.dtype), fam,

@@ -841,22 +841,24 @@ __global__ __launch_bounds__(kThreads) void k_findk(const SegDesc* __restrict__ 
 // buckets into their children's ranges (each chunk owns a private slice of
 // every child: its running offsets start at the chunk's row entries).  Every
 // key is bucketed exactly as in pass 1 (LUT in static LDS) and tested in the
-// active-bucket bitmap; per round a lane spills its keys to LDS and lists the
-// indices of its active keys (slots from one wavefront scan); the wavefront
-// then settles the listed keys 64 at a time: bucket, child, LDS cursor,
-// store.
+// active-bucket bitmap; the active keys of a round are appended straight
+// from registers to a work-group buffer in LDS (one LDS atomic per
+// wavefront).  The main loop issues no global stores, so its waits cover
+// only its own loads.  After each round the work-group synchronises; when
+// the buffer could not take another round it is flushed: every buffered key
+// is placed (child, LDS cursor, store) in one burst, so the keys of a child's
+// slice are written close together in time.
 constexpr uint32_t kWaves = kThreads / 64;
-constexpr int kRound = 4;                   // 16-byte loads per lane per round
-constexpr uint32_t kCompactCursors = 1024;  // LDS cursors; children beyond use global atomics
+constexpr int kRound = 4;                     // 16-byte loads per lane per round
+constexpr uint32_t kCompactCursors = 1024;    // LDS cursors; children beyond use global atomics
+constexpr uint32_t kCompactBufBytes = 96 * 1024;  // buffered active keys
 
 template <typename KT>
 __host__ __device__ constexpr size_t compact_lds_bytes(uint32_t pbits, uint32_t bmax) {
-  // dynamic part: cursors, the round's keys of every lane (64 B each) and
-  // the staged key indices (the LUT and bitmaps are static)
+  // dynamic part: cursors and the key buffer (the LUT and bitmaps are static)
   (void)pbits;
   (void)bmax;
-  return (size_t)kCompactCursors * 4 + (size_t)kThreads * kRound * 16 +
-         (size_t)kThreads * kRound * (16 / sizeof(KT)) * 2;
+  return (size_t)kCompactCursors * 4 + kCompactBufBytes;
 }
 
 template <typename KT, int C>
@@ -875,6 +877,7 @@ __global__ __launch_bounds__(kThreads) void k_compact(const KT* __restrict__ src
   __shared__ uint32_t lut[1u << kMaxPrefixBits];
   __shared__ uint32_t abits[kMaxBuckets / 32];  // active buckets
   __shared__ uint32_t wbase[kMaxBuckets / 32];  // first child of each bitmap word
+  __shared__ uint32_t nbuf;                     // keys in the buffer
   const Chunk ch = chunks[blockIdx.x];
   const SegDesc& sd = segs[ch.seg];
   const SegState st = states[ch.seg];
@@ -882,11 +885,14 @@ __global__ __launch_bounds__(kThreads) void k_compact(const KT* __restrict__ src
   if (U == 0) return;
   constexpr uint32_t VEC = 16 / sizeof(KT);
   constexpr int N = kRound * VEC;  // keys per lane per round
+  constexpr uint32_t CAP = kCompactBufBytes / sizeof(KT);
+  constexpr uint32_t ROUND_MAX = kThreads * N;  // most keys one round can append
+  // a round starts with at most CAP - ROUND_MAX keys buffered, so it fits
+  static_assert(CAP > ROUND_MAX, "buffer must hold more than a round");
   const uint32_t nwords = (st.B + 31) / 32;
   uint32_t* cur = lds;
-  uint4* kbuf = reinterpret_cast<uint4*>(cur + kCompactCursors);  // kRound per lane
-  uint16_t* sidx = reinterpret_cast<uint16_t*>(kbuf + kThreads * kRound);
-  const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = lane_id();
+  KT* kb = reinterpret_cast<KT*>(cur + kCompactCursors);
+  const uint32_t tid = threadIdx.x, lane = lane_id();
   strided_for(lut_pool + sd.lut_off, st.P, [&](uint32_t i, uint32_t v) { lut[i] = v; });
   for (uint32_t i = tid; i < nwords; i += kThreads) {
     abits[i] = bits_pool[sd.bits_off + i];
@@ -896,11 +902,12 @@ __global__ __launch_bounds__(kThreads) void k_compact(const KT* __restrict__ src
   const Child* kids = children + sd.q_off;
   for (uint32_t u = tid; u < U && u < kCompactCursors; u += kThreads)
     cur[u] = (uint32_t)kids[u].start + row[kids[u].bucket];
+  if (tid == 0) nbuf = 0;
   __syncthreads();
   BucketFn<KT> bk;
   bk.init(st);
-  // stores a key into its child's range if its bucket is active
-  auto settle = [&](KT raw) {
+  // moves a key of an active bucket to its child's slice
+  auto place = [&](KT raw) {
     const uint32_t b = bk(to_key<C>(raw), lut);
     const uint32_t bits = abits[b >> 5];
     const uint32_t sh = b & 31u;
@@ -912,6 +919,13 @@ __global__ __launch_bounds__(kThreads) void k_compact(const KT* __restrict__ src
       red[pos] = raw;
     }
   };
+  // every thread, with the same n: place the buffered keys, empty the buffer
+  auto flush = [&](uint32_t n) {
+    for (uint32_t i = tid; i < n; i += kThreads) place(kb[i]);
+    __syncthreads();
+    if (tid == 0) nbuf = 0;
+    __syncthreads();
+  };
   const KT* p = src + sd.data_off + ch.start;
   const uint64_t len = ch.len;
   uint64_t mis = ((uintptr_t)p & 15u) / sizeof(KT);
@@ -920,21 +934,16 @@ __global__ __launch_bounds__(kThreads) void k_compact(const KT* __restrict__ src
   const uint64_t nvec = (len - head) / VEC;
   const uint64_t done = head + nvec * VEC;
   // head and tail: fewer than VEC keys each (a thread may own one of each)
-  if (tid < head) settle(p[tid]);
-  if (done + tid < len) settle(p[done + tid]);
+  if (tid < head) place(p[tid]);
+  if (done + tid < len) place(p[done + tid]);
   const uint4* __restrict__ pv = reinterpret_cast<const uint4*>(p + head);
-  // lane-interleaved spill: wavefront w, load t, lane l at kbuf[(w*kRound + t)*64 + l],
-  // so a 16-byte LDS write of the whole wavefront is one contiguous KiB
-  uint4* my_keys = kbuf + (size_t)wave * 64 * kRound + lane;
-  const KT* wave_keys = reinterpret_cast<const KT*>(kbuf + (size_t)wave * 64 * kRound);
-  uint16_t* wave_idx = sidx + (size_t)wave * 64 * N;
   const uint64_t per_round = (uint64_t)kRound * kThreads;
-  const uint64_t rounds = (nvec + per_round - 1) / per_round;
-  // software pipeline: the loads of round r + 1 are in flight while the
-  // candidates of round r settle
+  const uint64_t rounds = (nvec + per_round - 1) / per_round;  // uniform
+  const uint64_t full_rounds = nvec / per_round;  // rounds with every load in range
+  // software pipeline: the loads of round r + 1 are in flight while round r
+  // is filtered and appended
   uint4 v[kRound];
   uint32_t okmask = 0;
-  const uint64_t full_rounds = nvec / per_round;  // rounds with every load in range
   auto load_round = [&](uint64_t r) {
     const uint4* q = pv + r * per_round + tid;
     if (r < full_rounds) {  // uniform
@@ -965,27 +974,23 @@ __global__ __launch_bounds__(kThreads) void k_compact(const KT* __restrict__ src
     const uint32_t inc = wave_incl_scan(cl);
     const uint32_t tot = wave_last(inc);  // uniform
     if (tot) {
-      // this lane's keys, then the indices of its candidates
+      uint32_t wb = 0;
+      if (lane == 0) wb = atomicAdd(&nbuf, tot);
+      uint32_t slot = (uint32_t)__builtin_amdgcn_readfirstlane((int)wb) + inc - cl;
 #pragma unroll
-      for (int t = 0; t < kRound; ++t) my_keys[t * 64] = v[t];
-      // key i of this lane sits in load t = i / VEC, component i % VEC
-      uint32_t slot = inc - cl;
-      for (uint32_t m = mask; m; m &= m - 1) {
-        const uint32_t i = __ffs(m) - 1;
-        wave_idx[slot++] = (uint16_t)(((i / VEC) * 64 + lane) * VEC + (i % VEC));
+      for (int i = 0; i < N; ++i) {
+        if ((mask >> i) & 1u) kb[slot++] = k[i];
       }
     }
     if (r + 1 < rounds) load_round(r + 1);
-    if (tot) {
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      for (uint32_t j = lane; j < tot; j += 64) settle(wave_keys[wave_idx[j]]);
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    }
+    __syncthreads();
+    const uint32_t nb = nbuf;
+    __syncthreads();  // every thread has read nbuf before the next round appends
+    if (nb > CAP - ROUND_MAX) flush(nb);  // uniform
   }
+  const uint32_t nb = nbuf;  // stable: nothing appends any more
+  __syncthreads();
+  flush(nb);
 }
 
 // Plain streaming copy of nv 16-byte words (grid-stride).

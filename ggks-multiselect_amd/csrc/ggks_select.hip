@@ -678,8 +678,10 @@ int ggks_tuning_reset(void) {
 int ggks_generate(void* d_ptr, int dtype, int family, uint64_t seed, uint64_t start, uint64_t n,
                   int64_t lo, int64_t hi) {
   if (!d_ptr && n) return fail(GGKS_EINVAL, "null pointer");
-  if (dtype < 0 || dtype > 3 || family < 0 || family > 2)
+  if (dtype < 0 || dtype > 3 || family < 0 || family > 4)
     return fail(GGKS_EINVAL, "bad dtype %d or family %d", dtype, family);
+  if (family >= 3 && (dtype == GGKS_U32 || dtype == GGKS_I32))
+    return fail(GGKS_EINVAL, "family %d is for float dtypes", family);
   if (family == 2 && (hi < lo || lo < 0 || hi > 0xFFFFFFFFll))
     return fail(GGKS_EINVAL, "bad levels range [%lld, %lld]", (long long)lo, (long long)hi);
   if ((dtype == GGKS_U32 || dtype == GGKS_I32) &&

@@ -294,7 +294,10 @@ int ggks_server_destroy(ggks_server *server);
  * (bit-identical to the Python package's host generators); family 1 =
  * normal(0,1) by Box-Muller in double (f32/f64; the device's libm may differ
  * from the host's in the last bit); family 2 = levels: f32/f64 holding the
- * integers lo..hi (0 <= lo <= hi < 2^32), heavy duplicates.  Synchronous. */
+ * integers lo..hi (0 <= lo <= hi < 2^32), heavy duplicates; family 3 =
+ * Cauchy(0,1), tan(pi (u - 1/2)) clipped to finite values (f32/f64);
+ * family 4 = plateau: 0.5 with probability 1/2, else uniform [0,1)
+ * (generateProblems.cu's adversarial shapes).  Synchronous. */
 int ggks_generate(void *d_ptr, int dtype, int family, uint64_t seed, uint64_t start, uint64_t n,
                   int64_t lo, int64_t hi);
 
