@@ -49,6 +49,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-looped leg")
+    ap.add_argument("--no-f64", action="store_true", help="skip the 2^27 float64 leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--serve-seconds", type=float, default=20.0,
                     help="length of each served trace (the workload definition uses 60)")
@@ -194,6 +195,34 @@ def batched_leg(pkg, L, nseg: int):
     }
 
 
+def f64_leg(pkg, L, steps: int):
+    """The headline configuration's float64 size: n = 2^27 N(0,1) (seed 2),
+    1024 uniform k, vector resident in HBM; mean of HIP-event step times."""
+    lib = L.lib()
+    n = 1 << 27
+    d = pkg.generate(np.float64, "normal", 2, n)
+    k = pkg.workloads.k_uniform(101, n, 1024)
+    dk = pkg.DeviceArray.from_host(k)
+    dout = pkg.DeviceArray(k.size, np.float64)
+    ctx = pkg.Context(0)
+    for _ in range(3):
+        pkg.multiselect_async(d, dk, dout, ctx)
+    L.check(lib.ggks_sync())
+    L.check(lib.ggks_ctx_profile_arm(ctx.handle, steps))
+    for _ in range(steps):
+        pkg.multiselect_async(d, dk, dout, ctx)
+    L.check(lib.ggks_sync())
+    ms = np.zeros(steps * (L.NSTAGES + 1), np.float32)
+    L.check(lib.ggks_ctx_profile_read(ctx.handle, ms.ctypes.data, steps))
+    ms = ms.reshape(steps, -1)
+    step_ms = float(ms[:, L.NSTAGES].mean())
+    return {"workload": f"bucket multiselect, n=2^27 f64 N(0,1), {k.size} uniform k",
+            "ms_per_step": round(step_ms, 4), "gelem_per_s": round(n / step_ms / 1e6, 2),
+            "hbm_frac_call": round(2 * n * 8 / (step_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "stage_us": {nm: round(float(ms[:, i].mean()) * 1e3, 2)
+                         for i, nm in enumerate(L.STAGES)}}
+
+
 def serving_leg(pkg, rates, seconds: float):
     """Served request streams (Poisson arrivals) over a pool of 64 resident
     vectors (length log-uniform [2^12, 2^26], dtype f32:u32:f64 = 6:3:1,
@@ -285,7 +314,9 @@ def main():
         with open(tpath) as f:
             tt = json.load(f)
         rec = tt.get(f"{a.kind}_{a.dtype}_2^{a.log2n}_{k.size}k", {}).get("kernels", {})
-        kname = "k_count" if dom == "count" else "k_compact"
+        conv = {"f32": 2, "f64": 2, "u32": 0, "i32": 1}[a.dtype]
+        key_t = "unsigned int" if esz == 4 else "unsigned long"
+        kname = f"{'k_count' if dom == 'count' else 'k_compact'}<{key_t}, {conv}>"
         if kname in rec:
             traffic = rec[kname]["bytes"]
 
@@ -327,6 +358,9 @@ def main():
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(pkg, x, a.numk, a.cpu_seconds)
     del d, x
+    if not a.no_f64:
+        print("# f64 leg", file=sys.stderr, flush=True)
+        out["f64"] = f64_leg(pkg, L, a.steps)
     if not a.no_batch:
         print("# batched leg", file=sys.stderr, flush=True)
         out["batched"] = batched_leg(pkg, L, a.batch_segments)

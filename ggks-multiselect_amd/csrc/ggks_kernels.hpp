@@ -694,11 +694,6 @@ __global__ __launch_bounds__(kThreads) void k_findk(const SegDesc* __restrict__ 
   // table loads and scans
   const uint32_t kfirst =
       (sd.qsrc == QSRC_K && threadIdx.x < sd.q_cnt) ? kraw[sd.q_off + threadIdx.x] : 0u;
-  __shared__ SegState st;
-  if (threadIdx.x == 0) st = states[blockIdx.x];
-  __syncthreads();
-  const uint32_t B = st.B;
-  const uint32_t nwords = (B + 31) / 32;
   const uint32_t wcap = (sd.bmax + 31) / 32;
   uint32_t* hist = lds;                          // bmax
   uint32_t* lut = hist + sd.bmax;                // 2^pbits
@@ -707,11 +702,19 @@ __global__ __launch_bounds__(kThreads) void k_findk(const SegDesc* __restrict__ 
   uint32_t* wc = wb + wcap;                      // wcap: first reduced slot of a word
   uint32_t* wsum = wc + wcap;                    // 17 (+ pad)
   const uint32_t tid = threadIdx.x;
-  // bucket prefixes inside groups of kReduceBins, and the group sums
+  // bucket prefixes inside groups of kReduceBins, the group sums and the LUT:
+  // they need only the descriptor, so their loads go out before the state
+  // (entries past the segment's bucket count are loaded and never used)
+  strided_for(totals + sd.tot_off, sd.bmax, [&](uint32_t i, uint32_t v) { hist[i] = v; });
+  strided_for(gsums + sd.tot_off / kReduceBins, sd.bmax / kReduceBins,
+              [&](uint32_t i, uint32_t v) { wc[i] = v; });
+  strided_for(lut_pool + sd.lut_off, 1u << sd.pbits, [&](uint32_t i, uint32_t v) { lut[i] = v; });
+  __shared__ SegState st;
+  if (threadIdx.x == 0) st = states[blockIdx.x];
+  __syncthreads();
+  const uint32_t B = st.B;
+  const uint32_t nwords = (B + 31) / 32;
   const uint32_t ngroups = (B + kReduceBins - 1) / kReduceBins;  // <= wcap
-  strided_for(totals + sd.tot_off, B, [&](uint32_t i, uint32_t v) { hist[i] = v; });
-  strided_for(gsums + sd.tot_off / kReduceBins, ngroups, [&](uint32_t i, uint32_t v) { wc[i] = v; });
-  strided_for(lut_pool + sd.lut_off, st.P, [&](uint32_t i, uint32_t v) { lut[i] = v; });
   for (uint32_t i = tid; i < nwords; i += kThreads) bits[i] = 0;
   __syncthreads();
   const uint32_t len = block_scan_array(wc, ngroups, wsum);  // group bases
