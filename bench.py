@@ -249,7 +249,7 @@ def main():
     a = parse()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported "
-              "(multi-GPU sharding is described in DESIGN.md, not built)", file=sys.stderr)
+              "(the path runs on one GPU; sharding across GPUs is not built)", file=sys.stderr)
         sys.exit(2)
     pkg = load_package()
     from ggks_multiselect_amd import _lib as L  # noqa: E402

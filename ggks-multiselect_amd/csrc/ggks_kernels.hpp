@@ -27,6 +27,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "ggks_keys.hpp"
 #include "ggks_types.hpp"
 
@@ -182,6 +184,23 @@ struct BucketFn {
       t = ((uint32_t)(l64 >> rsh) & ((1u << twid) - 1u)) << lsh;
     }
   }
+  // stage 1 with one shift for t: finish() multiplies only the low 24 bits
+  // of t, and those bits of low << lsh (sub-fields of s <= 24 bits, rsh = 0)
+  // or of low >> rsh (s > 24, lsh = 0) equal the normalised sub-field, so
+  // the bucket is the same as split()'s.  LEFT must be s <= 24.
+  template <bool LEFT>
+  __device__ __forceinline__ void split1(KT key, uint32_t& p, uint32_t& t) const {
+    KT low = (key > kb ? key : kb) - kb;
+    low = low < lmax ? low : lmax;
+    if (sizeof(KT) == 4) {
+      p = __builtin_amdgcn_ubfe((uint32_t)low, poff, pwid);
+      t = LEFT ? (uint32_t)low << lsh : (uint32_t)low >> rsh;
+    } else {
+      const uint64_t l64 = (uint64_t)low;
+      p = (uint32_t)(l64 >> poff) & ((1u << pwid) - 1u);
+      t = LEFT ? (uint32_t)l64 << lsh : (uint32_t)(l64 >> rsh);
+    }
+  }
   // stage 2: bucket from the LUT entry e = lut[p]
   __device__ __forceinline__ uint32_t finish(uint32_t e, uint32_t t) const {
     const uint32_t nbsh = (e >> 16) << 8;  // nb * 2^8 < 2^24
@@ -209,6 +228,21 @@ __device__ __forceinline__ void bucket_batch(const BucketFn<KT>& fn, const uint3
   uint32_t p[N], t[N], e[N];
 #pragma unroll
   for (int i = 0; i < N; ++i) fn.split(to_key<C>(raw[i]), p[i], t[i]);
+#pragma unroll
+  for (int i = 0; i < N; ++i) e[i] = lut[p[i]];
+#pragma unroll
+  for (int i = 0; i < N; ++i) b[i] = fn.finish(e[i], t[i]);
+}
+
+// bucket_batch for the streaming passes: split1 with the shift direction
+// fixed at compile time (each pass runs its loop in the instance chosen by
+// the segment's s, so the loop body has no branch).
+template <typename KT, int C, int N, bool LEFT>
+__device__ __forceinline__ void bucket_batch1(const BucketFn<KT>& fn, const uint32_t* lut,
+                                              const KT (&raw)[N], uint32_t (&b)[N]) {
+  uint32_t p[N], t[N], e[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) fn.template split1<LEFT>(to_key<C>(raw[i]), p[i], t[i]);
 #pragma unroll
   for (int i = 0; i < N; ++i) e[i] = lut[p[i]];
 #pragma unroll
@@ -558,17 +592,21 @@ __global__ __launch_bounds__(kThreads) void k_count(const KT* __restrict__ src,
   const uint4* __restrict__ pv = reinterpret_cast<const uint4*>(p + head);
   const uint64_t per_round = (uint64_t)R * kThreads;
   uint64_t r0 = 0;
-  for (; r0 + per_round <= nvec; r0 += per_round) {  // full rounds
-    uint4 v[R];
+  auto full_rounds = [&](auto left) {
+    for (; r0 + per_round <= nvec; r0 += per_round) {
+      uint4 v[R];
 #pragma unroll
-    for (int t = 0; t < R; ++t) v[t] = stream_load(&pv[r0 + (uint64_t)t * kThreads + tid]);
-    KT k[N];
-    unpack<KT, R>(v, k);
-    uint32_t b[N];
-    bucket_batch<KT, C, N>(bk, lut, k, b);
+      for (int t = 0; t < R; ++t) v[t] = stream_load(&pv[r0 + (uint64_t)t * kThreads + tid]);
+      KT k[N];
+      unpack<KT, R>(v, k);
+      uint32_t b[N];
+      bucket_batch1<KT, C, N, decltype(left)::value>(bk, lut, k, b);
 #pragma unroll
-    for (int i = 0; i < N; ++i) atomicAdd(&hist[b[i]], 1u);
-  }
+      for (int i = 0; i < N; ++i) atomicAdd(&hist[b[i]], 1u);
+    }
+  };
+  if (st.s <= 24) full_rounds(std::true_type{});  // uniform: the sub-field shift direction
+  else full_rounds(std::false_type{});
   for (uint64_t i = r0 + tid; i < nvec; i += kThreads) {  // last partial round
     uint4 v[1] = {stream_load(&pv[i])};
     KT k[VEC];
@@ -688,7 +726,7 @@ __global__ __launch_bounds__(kThreads) void k_findk(const SegDesc* __restrict__ 
                                                     uint32_t* __restrict__ bits_pool,
                                                     uint32_t* __restrict__ wbase_pool,
                                                     KT* __restrict__ out) {
-  extern __shared__ uint32_t lds[];
+  extern __shared__ __align__(16) uint32_t lds[];
   const SegDesc sd = segs[blockIdx.x];
   // the first k of this thread, read now so its latency hides under the
   // table loads and scans
@@ -703,22 +741,40 @@ __global__ __launch_bounds__(kThreads) void k_findk(const SegDesc* __restrict__ 
   uint32_t* wsum = wc + wcap;                    // 17 (+ pad)
   const uint32_t tid = threadIdx.x;
   // bucket prefixes inside groups of kReduceBins, the group sums and the LUT:
-  // they need only the descriptor, so their loads go out before the state
-  // (entries past the segment's bucket count are loaded and never used)
-  strided_for(totals + sd.tot_off, sd.bmax, [&](uint32_t i, uint32_t v) { hist[i] = v; });
-  strided_for(gsums + sd.tot_off / kReduceBins, sd.bmax / kReduceBins,
-              [&](uint32_t i, uint32_t v) { wc[i] = v; });
-  strided_for(lut_pool + sd.lut_off, 1u << sd.pbits, [&](uint32_t i, uint32_t v) { lut[i] = v; });
+  // they need only the descriptor, so their loads (16 bytes each, all of a
+  // thread's at once) go out before the state.  The table sizes are
+  // multiples of 4 words at 16-byte aligned offsets.  The bucket prefixes
+  // stay in registers until the group bases are scanned and are stored once,
+  // base added.  Entries past the segment's bucket count are never used.
+  constexpr uint32_t TV = kMaxBuckets / 4 / kThreads;
+  const uint32_t nt4 = sd.bmax / 4, ng4 = sd.bmax / kReduceBins / 4, nl4 = (1u << sd.pbits) / 4;
+  const uint4* tot4 = reinterpret_cast<const uint4*>(totals + sd.tot_off);
+  uint4 tv[TV];
+#pragma unroll
+  for (uint32_t u = 0; u < TV; ++u)
+    if (tid + u * kThreads < nt4) tv[u] = tot4[tid + u * kThreads];
+  uint4 gv = make_uint4(0, 0, 0, 0), lv = make_uint4(0, 0, 0, 0);
+  if (tid < ng4) gv = reinterpret_cast<const uint4*>(gsums + sd.tot_off / kReduceBins)[tid];
+  if (tid < nl4) lv = reinterpret_cast<const uint4*>(lut_pool + sd.lut_off)[tid];
   __shared__ SegState st;
   if (threadIdx.x == 0) st = states[blockIdx.x];
+  if (tid < ng4) reinterpret_cast<uint4*>(wc)[tid] = gv;
+  if (tid < nl4) reinterpret_cast<uint4*>(lut)[tid] = lv;
   __syncthreads();
   const uint32_t B = st.B;
   const uint32_t nwords = (B + 31) / 32;
   const uint32_t ngroups = (B + kReduceBins - 1) / kReduceBins;  // <= wcap
   for (uint32_t i = tid; i < nwords; i += kThreads) bits[i] = 0;
-  __syncthreads();
-  const uint32_t len = block_scan_array(wc, ngroups, wsum);  // group bases
-  for (uint32_t i = tid; i < B; i += kThreads) hist[i] += wc[i / kReduceBins];
+  const uint32_t len = block_scan_array(wc, ngroups, wsum);  // group bases (synchronises)
+#pragma unroll
+  for (uint32_t u = 0; u < TV; ++u) {
+    const uint32_t i4 = tid + u * kThreads;
+    if (i4 < nt4) {
+      const uint32_t g = wc[i4 * 4 / kReduceBins];
+      reinterpret_cast<uint4*>(hist)[i4] =
+          make_uint4(tv[u].x + g, tv[u].y + g, tv[u].z + g, tv[u].w + g);
+    }
+  }
   __syncthreads();  // hist = exclusive prefix of the bucket counts
   RangeOf<KT> range;
   range.fn.init(st);
@@ -843,10 +899,10 @@ __global__ __launch_bounds__(kThreads) void k_findk(const SegDesc* __restrict__ 
 // Pass 2 (k_compact): stream the chunk once and move the keys of active
 // buckets into their children's ranges (each chunk owns a private slice of
 // every child: its running offsets start at the chunk's row entries).  Every
-// key is bucketed exactly as in pass 1 (LUT in static LDS) and tested in the
-// active-bucket bitmap; the active keys of a round are appended straight
-// from registers to a work-group buffer in LDS (one LDS atomic per
-// wavefront).  The main loop issues no global stores, so its waits cover
+// key is bucketed exactly as in pass 1 (LUT in static LDS) and tested in a
+// byte-per-bucket copy of the active-bucket bitmap (one LDS byte load, no bit
+// arithmetic); the active keys of a round are appended straight from
+// registers to a work-group buffer in LDS (one LDS atomic per wavefront).  The main loop issues no global stores, so its waits cover
 // only its own loads.  After each round the work-group synchronises; when
 // the buffer could not take another round it is flushed: every buffered key
 // is placed (child, LDS cursor, store) in one burst, so the keys of a child's
@@ -880,6 +936,7 @@ __global__ __launch_bounds__(kThreads) void k_compact(const KT* __restrict__ src
   __shared__ uint32_t lut[1u << kMaxPrefixBits];
   __shared__ uint32_t abits[kMaxBuckets / 32];  // active buckets
   __shared__ uint32_t wbase[kMaxBuckets / 32];  // first child of each bitmap word
+  __shared__ uint32_t abyte[kMaxBuckets / 4];   // active buckets, a byte (0 or 1) each
   __shared__ uint32_t nbuf;                     // keys in the buffer
   const Chunk ch = chunks[blockIdx.x];
   const SegDesc& sd = segs[ch.seg];
@@ -898,9 +955,15 @@ __global__ __launch_bounds__(kThreads) void k_compact(const KT* __restrict__ src
   const uint32_t tid = threadIdx.x, lane = lane_id();
   strided_for(lut_pool + sd.lut_off, st.P, [&](uint32_t i, uint32_t v) { lut[i] = v; });
   for (uint32_t i = tid; i < nwords; i += kThreads) {
-    abits[i] = bits_pool[sd.bits_off + i];
+    const uint32_t w = bits_pool[sd.bits_off + i];
+    abits[i] = w;
     wbase[i] = wbase_pool[sd.bits_off + i];
+    // bits 4j..4j+3 -> bytes of abyte[8i + j]: x * 0x204081 puts bit m of a
+    // nibble x at bit 8m, and no two shifted copies overlap
+#pragma unroll
+    for (uint32_t j = 0; j < 8; ++j) abyte[i * 8 + j] = (((w >> (4 * j)) & 0xFu) * 0x204081u) & 0x01010101u;
   }
+  const uint8_t* const act_of = reinterpret_cast<const uint8_t*>(abyte);
   uint32_t* row = rows + sd.row_base + (uint64_t)ch.idx * sd.bmax;
   const Child* kids = children + sd.q_off;
   for (uint32_t u = tid; u < U && u < kCompactCursors; u += kThreads)
@@ -922,9 +985,31 @@ __global__ __launch_bounds__(kThreads) void k_compact(const KT* __restrict__ src
       red[pos] = raw;
     }
   };
-  // every thread, with the same n: place the buffered keys, empty the buffer
+  // every thread, with the same n: place the buffered keys, empty the buffer.
+  // Buffered keys are active by construction; F of them per thread go
+  // through each step together so their LDS round trips overlap.
   auto flush = [&](uint32_t n) {
-    for (uint32_t i = tid; i < n; i += kThreads) place(kb[i]);
+    constexpr int F = 8;
+    uint32_t i = tid;
+    for (; i + (F - 1) * kThreads < n; i += F * kThreads) {
+      KT kk[F];
+      uint32_t bb[F], w[F], u[F];
+#pragma unroll
+      for (int f = 0; f < F; ++f) kk[f] = kb[i + f * kThreads];
+      bucket_batch<KT, C, F>(bk, lut, kk, bb);
+#pragma unroll
+      for (int f = 0; f < F; ++f) w[f] = abits[bb[f] >> 5];
+#pragma unroll
+      for (int f = 0; f < F; ++f) u[f] = wbase[bb[f] >> 5] + __popc(w[f] & ((1u << (bb[f] & 31u)) - 1u));
+#pragma unroll
+      for (int f = 0; f < F; ++f) {
+        const uint32_t pos = u[f] < kCompactCursors
+                                 ? atomicAdd(&cur[u[f]], 1u)
+                                 : (uint32_t)kids[u[f]].start + atomicAdd(&row[kids[u[f]].bucket], 1u);
+        red[pos] = kk[f];
+      }
+    }
+    for (; i < n; i += kThreads) place(kb[i]);
     __syncthreads();
     if (tid == 0) nbuf = 0;
     __syncthreads();
@@ -965,32 +1050,41 @@ __global__ __launch_bounds__(kThreads) void k_compact(const KT* __restrict__ src
     }
   };
   if (rounds) load_round(0);
-  for (uint64_t r = 0; r < rounds; ++r) {
-    KT k[N];
-    unpack<KT, kRound>(v, k);
-    uint32_t b[N], mask = 0;
-    bucket_batch<KT, C, N>(bk, lut, k, b);
+  auto main_loop = [&](auto left) {
+    for (uint64_t r = 0; r < rounds; ++r) {
+      KT k[N];
+      unpack<KT, kRound>(v, k);
+      uint32_t b[N], act[N];
+      bucket_batch1<KT, C, N, decltype(left)::value>(bk, lut, k, b);
 #pragma unroll
-    for (int i = 0; i < N; ++i) mask |= bitmap_test(abits, b[i]) << i;
-    mask &= okmask;
-    const uint32_t cl = __popc(mask);
-    const uint32_t inc = wave_incl_scan(cl);
-    const uint32_t tot = wave_last(inc);  // uniform
-    if (tot) {
-      uint32_t wb = 0;
-      if (lane == 0) wb = atomicAdd(&nbuf, tot);
-      uint32_t slot = (uint32_t)__builtin_amdgcn_readfirstlane((int)wb) + inc - cl;
+      for (int i = 0; i < N; ++i) act[i] = act_of[b[i]];
+      if (r >= full_rounds) {  // uniform: the chunk's last, partial round
 #pragma unroll
-      for (int i = 0; i < N; ++i) {
-        if ((mask >> i) & 1u) kb[slot++] = k[i];
+        for (int i = 0; i < N; ++i) act[i] &= (okmask >> i) & 1u;
       }
+      uint32_t cl = 0;
+#pragma unroll
+      for (int i = 0; i < N; ++i) cl += act[i];
+      const uint32_t inc = wave_incl_scan(cl);
+      const uint32_t tot = wave_last(inc);  // uniform
+      if (tot) {
+        uint32_t wb = 0;
+        if (lane == 0) wb = atomicAdd(&nbuf, tot);
+        KT* dst = kb + ((uint32_t)__builtin_amdgcn_readfirstlane((int)wb) + inc - cl);
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+          if (act[i]) *dst++ = k[i];
+        }
+      }
+      if (r + 1 < rounds) load_round(r + 1);
+      __syncthreads();
+      const uint32_t nb = nbuf;
+      __syncthreads();  // every thread has read nbuf before the next round appends
+      if (nb > CAP - ROUND_MAX) flush(nb);  // uniform
     }
-    if (r + 1 < rounds) load_round(r + 1);
-    __syncthreads();
-    const uint32_t nb = nbuf;
-    __syncthreads();  // every thread has read nbuf before the next round appends
-    if (nb > CAP - ROUND_MAX) flush(nb);  // uniform
-  }
+  };
+  if (st.s <= 24) main_loop(std::true_type{});  // uniform: the sub-field shift direction
+  else main_loop(std::false_type{});
   const uint32_t nb = nbuf;  // stable: nothing appends any more
   __syncthreads();
   flush(nb);
@@ -1051,6 +1145,59 @@ __device__ __forceinline__ void group_for_each(const KT* __restrict__ p, uint64_
   if (done + tid < len) f(p[done + tid]);
 }
 
+// A child's keys held in registers by a work-group of kFThreads: R 16-byte
+// words per thread plus at most one unaligned head key and one tail key, so
+// a child of up to R * kFThreads * 16 bytes is read from memory once for all
+// passes.  Visiting order differs from group_for_each; no pass depends on it.
+template <typename KT, int R>
+struct GroupKeys {
+  static constexpr uint32_t VEC = 16 / sizeof(KT);
+  uint4 v[R];
+  KT hk, tk;
+  uint32_t nv;  // valid words of this thread: v[0, nv)
+  bool hh, ht;
+  // whether a child at p of len keys fits (uniform)
+  __device__ static bool fits(const KT* p, uint64_t len) {
+    const uint64_t mis = ((uintptr_t)p & 15u) / sizeof(KT);
+    uint64_t head = mis ? (VEC - mis) : 0;
+    if (head > len) head = len;
+    return (len - head) / VEC <= (uint64_t)R * kFThreads;
+  }
+  __device__ void load(const KT* __restrict__ p, uint64_t len) {
+    const uint32_t tid = threadIdx.x;
+    const uint64_t mis = ((uintptr_t)p & 15u) / sizeof(KT);
+    uint64_t head = mis ? (VEC - mis) : 0;
+    if (head > len) head = len;
+    const uint32_t nvec = (uint32_t)((len - head) / VEC);  // <= R * kFThreads
+    const uint4* __restrict__ pv = reinterpret_cast<const uint4*>(p + head);
+    nv = nvec > tid ? (nvec - tid + kFThreads - 1) / kFThreads : 0u;
+#pragma unroll
+    for (int u = 0; u < R; ++u) v[u] = (uint32_t)u < nv ? pv[tid + u * kFThreads] : make_uint4(0, 0, 0, 0);
+    hh = tid < head;
+    hk = hh ? p[tid] : KT(0);
+    const uint64_t done = head + (uint64_t)nvec * VEC;
+    ht = done + tid < len;
+    tk = ht ? p[done + tid] : KT(0);
+  }
+  template <typename F>
+  __device__ __forceinline__ void for_each(F&& f) const {
+    if (hh) f(hk);
+#pragma unroll
+    for (int u = 0; u < R; ++u) {
+      if ((uint32_t)u < nv) {
+        if (sizeof(KT) == 4) {
+          f((KT)v[u].x); f((KT)v[u].y); f((KT)v[u].z); f((KT)v[u].w);
+        } else {
+          f((KT)(((uint64_t)v[u].y << 32) | v[u].x));
+          f((KT)(((uint64_t)v[u].w << 32) | v[u].z));
+        }
+      }
+    }
+    if (ht) f(tk);
+  }
+};
+constexpr int kFRegWords = 10;  // 16-byte words per thread: children up to 40 KB
+
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -1086,7 +1233,7 @@ __device__ __forceinline__ void wave_pick256(const uint32_t* h, uint32_t lane, u
 // radix passes over LDS up to the gather capacity, and beyond it (heavy
 // duplicates of one digit) by 8-bit radix passes over the child in memory.
 template <typename KT, int C>
-__global__ __launch_bounds__(kFThreads) void k_finish_small(const KT* __restrict__ input,
+__global__ __launch_bounds__(kFThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_finish_small(const KT* __restrict__ input,
                                                             const KT* __restrict__ red,
                                                             const Child* __restrict__ children,
                                                             uint32_t nchildren,
@@ -1105,157 +1252,168 @@ __global__ __launch_bounds__(kFThreads) void k_finish_small(const KT* __restrict
     const Child c = children[ci];
     if (c.q_cnt == 0 || !wave_child(c)) continue;  // uniform
     const KT* srcp = (c.src ? red : input) + c.start;
-    KT lo = (KT)c.lo, hi = (KT)c.hi;
-    if (c.need_range) {
-      KT a = KeyBits<KT>::MAX, b = 0;
-      group_for_each(srcp, c.len, [&](KT raw) {
-        const KT k = to_key<C>(raw);
-        a = k < a ? k : a;
-        b = k > b ? k : b;
-      });
-      for (int d = 32; d >= 1; d >>= 1) {
-        const KT x = __shfl_xor(a, d, 64), y = __shfl_xor(b, d, 64);
-        a = x < a ? x : a;
-        b = y > b ? y : b;
-      }
-      if (lane == 0) {
-        red_lo[tid >> 6] = a;
-        red_hi[tid >> 6] = b;
-      }
-      __syncthreads();
-      lo = red_lo[0];
-      hi = red_hi[0];
-      for (uint32_t w = 1; w < kFThreads / 64; ++w) {
-        lo = red_lo[w] < lo ? red_lo[w] : lo;
-        hi = red_hi[w] > hi ? red_hi[w] : hi;
-      }
-    }
-    const int nbits = bitlen<KT>((KT)(hi - lo));
-    if (nbits == 0) {  // every key equal
-      for (uint32_t qi = tid; qi < c.q_cnt; qi += kFThreads)
-        out[cq[c.q_off + qi].out] = from_key<C>(lo);
-      __syncthreads();
-      continue;
-    }
-    // pass 1: histogram of the top digit
-    const int d1 = nbits < (int)kFDigit ? nbits : (int)kFDigit;
-    const int sh1 = nbits - d1;
-    for (uint32_t j = tid; j < kFWords; j += kFThreads) h[j] = 0;
-    __syncthreads();
-    group_for_each(srcp, c.len, [&](KT raw) {
-      const uint32_t bn = (uint32_t)((KT)(to_key<C>(raw) - lo) >> sh1);
-      atomicAdd(&h[bn + (bn >> 5)], 1u);
-    });
-    __syncthreads();
-    if (w0) {  // each rank's top digit, residual rank and digit count
-      uint32_t lsum = 0;  // lane's bins lane*32 .. lane*32 + 31
-#pragma unroll 8
-      for (uint32_t j = 0; j < 32; ++j) lsum += h[lane * 33 + j];
-      const uint32_t lex = wave_incl_scan(lsum) - lsum;
-      for (uint32_t qi = 0; qi < c.q_cnt; ++qi) {
-        const uint32_t r = cq[c.q_off + qi].rank - 1;
-        const int L = __ffsll((unsigned long long)__ballot(r >= lex && r < lex + lsum)) - 1;
-        const uint32_t eL = (uint32_t)__shfl((int)lex, L, 64);
-        const uint32_t v = lane < 32 ? h[(uint32_t)L * 33 + lane] : 0u;
-        const uint32_t ex = wave_incl_scan(v) - v + eL;
-        const int j = __ffsll((unsigned long long)__ballot(lane < 32 && r >= ex && r < ex + v)) - 1;
-        const uint32_t exj = (uint32_t)__shfl((int)ex, j, 64);  // every lane takes part
-        const uint32_t vj = (uint32_t)__shfl((int)v, j, 64);
+    // the passes over the child's keys, given a visitor of its keys
+    auto body = [&](auto&& each) {
+      KT lo = (KT)c.lo, hi = (KT)c.hi;
+      if (c.need_range) {
+        KT a = KeyBits<KT>::MAX, b = 0;
+        each([&](KT raw) {
+          const KT k = to_key<C>(raw);
+          a = k < a ? k : a;
+          b = k > b ? k : b;
+        });
+        for (int d = 32; d >= 1; d >>= 1) {
+          const KT x = __shfl_xor(a, d, 64), y = __shfl_xor(b, d, 64);
+          a = x < a ? x : a;
+          b = y > b ? y : b;
+        }
         if (lane == 0) {
-          res[3 * qi] = (uint32_t)L * 32 + (uint32_t)j;
-          res[3 * qi + 1] = r - exj;
-          res[3 * qi + 2] = vj;
+          red_lo[tid >> 6] = a;
+          red_hi[tid >> 6] = b;
+        }
+        __syncthreads();
+        lo = red_lo[0];
+        hi = red_hi[0];
+        for (uint32_t w = 1; w < kFThreads / 64; ++w) {
+          lo = red_lo[w] < lo ? red_lo[w] : lo;
+          hi = red_hi[w] > hi ? red_hi[w] : hi;
         }
       }
-    }
-    __syncthreads();
-    const KT lowmask = sh1 == 0 ? KT(0) : (KT)(KeyBits<KT>::MAX >> (W - sh1));
-    for (uint32_t qi = 0; qi < c.q_cnt; ++qi) {
-      const Query q = cq[c.q_off + qi];
-      const KT top = (KT)res[3 * qi];
-      uint32_t r = res[3 * qi + 1];
-      const uint32_t n2 = res[3 * qi + 2];
-      KT low = 0;  // the rank's key minus lo, below the top digit
-      if (sh1 > 0 && n2 <= CAP) {
-        // pass 2: gather the keys with this top digit
-        if (tid == 0) res[4 * kWaveRanks] = 0;
-        __syncthreads();
-        group_for_each(srcp, c.len, [&](KT raw) {
-          const KT rel = (KT)(to_key<C>(raw) - lo);
-          if ((KT)(rel >> sh1) == top) buf[atomicAdd(&res[4 * kWaveRanks], 1u)] = rel & lowmask;
-        });
-        __syncthreads();
-        if (w0) {
-          if (n2 <= 64) {
-            // rank by counting: lane i holds key i; ties broken by position
-            const KT mine = lane < n2 ? buf[lane] : KeyBits<KT>::MAX;
-            uint32_t below = 0;
-            for (uint32_t j = 0; j < n2; ++j) {
-              const KT o = buf[j];
-              below += (o < mine || (o == mine && j < lane)) ? 1u : 0u;
-            }
-            const int src = __ffsll((unsigned long long)__ballot(lane < n2 && below == r)) - 1;
-            low = (KT)__shfl(mine, src, 64);
-          } else {
-            int rem = sh1;
-            KT pre = 0;
-            while (rem > 0) {
-              const int d = rem < 8 ? rem : 8;
-              const int shift = rem - d;
-              const int hshift = shift + d;
-#pragma unroll
-              for (int j = 0; j < 4; ++j) bins[lane * 4 + j] = 0;
-              wave_sync();
-              for (uint32_t i = lane; i < n2; i += 64) {
-                const KT v = buf[i];
-                const KT t = hshift >= (int)W ? KT(0) : (KT)(v >> hshift);
-                if (t == pre) atomicAdd(&bins[(uint32_t)(v >> shift) & ((1u << d) - 1u)], 1u);
-              }
-              wave_sync();
-              uint32_t bn, below;
-              wave_pick256(bins, lane, r, bn, below);
-              r -= below;
-              pre = (KT)((pre << d) | (KT)bn);
-              rem = shift;
-              wave_sync();
-            }
-            low = pre;
+      const int nbits = bitlen<KT>((KT)(hi - lo));
+      if (nbits == 0) {  // every key equal
+        for (uint32_t qi = tid; qi < c.q_cnt; qi += kFThreads)
+          out[cq[c.q_off + qi].out] = from_key<C>(lo);
+        return;
+      }
+      // pass 1: histogram of the top digit
+      const int d1 = nbits < (int)kFDigit ? nbits : (int)kFDigit;
+      const int sh1 = nbits - d1;
+      for (uint32_t j = tid; j < kFWords; j += kFThreads) h[j] = 0;
+      __syncthreads();
+      each([&](KT raw) {
+        const uint32_t bn = (uint32_t)((KT)(to_key<C>(raw) - lo) >> sh1);
+        atomicAdd(&h[bn + (bn >> 5)], 1u);
+      });
+      __syncthreads();
+      if (w0) {  // each rank's top digit, residual rank and digit count
+        uint32_t lsum = 0;  // lane's bins lane*32 .. lane*32 + 31
+  #pragma unroll 8
+        for (uint32_t j = 0; j < 32; ++j) lsum += h[lane * 33 + j];
+        const uint32_t lex = wave_incl_scan(lsum) - lsum;
+        for (uint32_t qi = 0; qi < c.q_cnt; ++qi) {
+          const uint32_t r = cq[c.q_off + qi].rank - 1;
+          const int L = __ffsll((unsigned long long)__ballot(r >= lex && r < lex + lsum)) - 1;
+          const uint32_t eL = (uint32_t)__shfl((int)lex, L, 64);
+          const uint32_t v = lane < 32 ? h[(uint32_t)L * 33 + lane] : 0u;
+          const uint32_t ex = wave_incl_scan(v) - v + eL;
+          const int j = __ffsll((unsigned long long)__ballot(lane < 32 && r >= ex && r < ex + v)) - 1;
+          const uint32_t exj = (uint32_t)__shfl((int)ex, j, 64);  // every lane takes part
+          const uint32_t vj = (uint32_t)__shfl((int)v, j, 64);
+          if (lane == 0) {
+            res[3 * qi] = (uint32_t)L * 32 + (uint32_t)j;
+            res[3 * qi + 1] = r - exj;
+            res[3 * qi + 2] = vj;
           }
         }
-        __syncthreads();
-      } else if (sh1 > 0) {
-        // many keys share the top digit: 8-bit passes over the child in memory
-        int rem = sh1;
-        KT pre = top;
-        while (rem > 0) {
-          const int d = rem < 8 ? rem : 8;
-          const int shift = rem - d;
-          const int hshift = shift + d;
-          if (tid < 256) bins[tid] = 0;
+      }
+      __syncthreads();
+      const KT lowmask = sh1 == 0 ? KT(0) : (KT)(KeyBits<KT>::MAX >> (W - sh1));
+      for (uint32_t qi = 0; qi < c.q_cnt; ++qi) {
+        const Query q = cq[c.q_off + qi];
+        const KT top = (KT)res[3 * qi];
+        uint32_t r = res[3 * qi + 1];
+        const uint32_t n2 = res[3 * qi + 2];
+        KT low = 0;  // the rank's key minus lo, below the top digit
+        if (sh1 > 0 && n2 <= CAP) {
+          // pass 2: gather the keys with this top digit
+          if (tid == 0) res[4 * kWaveRanks] = 0;
           __syncthreads();
-          group_for_each(srcp, c.len, [&](KT raw) {
+          each([&](KT raw) {
             const KT rel = (KT)(to_key<C>(raw) - lo);
-            if ((KT)(rel >> hshift) == pre)
-              atomicAdd(&bins[(uint32_t)(rel >> shift) & ((1u << d) - 1u)], 1u);
+            if ((KT)(rel >> sh1) == top) buf[atomicAdd(&res[4 * kWaveRanks], 1u)] = rel & lowmask;
           });
           __syncthreads();
           if (w0) {
-            uint32_t bn, below;
-            wave_pick256(bins, lane, r, bn, below);
-            if (lane == 0) {
-              res[4 * kWaveRanks + 1] = bn;
-              res[4 * kWaveRanks] = below;
+            if (n2 <= 64) {
+              // rank by counting: lane i holds key i; ties broken by position
+              const KT mine = lane < n2 ? buf[lane] : KeyBits<KT>::MAX;
+              uint32_t below = 0;
+              for (uint32_t j = 0; j < n2; ++j) {
+                const KT o = buf[j];
+                below += (o < mine || (o == mine && j < lane)) ? 1u : 0u;
+              }
+              const int src = __ffsll((unsigned long long)__ballot(lane < n2 && below == r)) - 1;
+              low = (KT)__shfl(mine, src, 64);
+            } else {
+              int rem = sh1;
+              KT pre = 0;
+              while (rem > 0) {
+                const int d = rem < 8 ? rem : 8;
+                const int shift = rem - d;
+                const int hshift = shift + d;
+  #pragma unroll
+                for (int j = 0; j < 4; ++j) bins[lane * 4 + j] = 0;
+                wave_sync();
+                for (uint32_t i = lane; i < n2; i += 64) {
+                  const KT v = buf[i];
+                  const KT t = hshift >= (int)W ? KT(0) : (KT)(v >> hshift);
+                  if (t == pre) atomicAdd(&bins[(uint32_t)(v >> shift) & ((1u << d) - 1u)], 1u);
+                }
+                wave_sync();
+                uint32_t bn, below;
+                wave_pick256(bins, lane, r, bn, below);
+                r -= below;
+                pre = (KT)((pre << d) | (KT)bn);
+                rem = shift;
+                wave_sync();
+              }
+              low = pre;
             }
           }
           __syncthreads();
-          r -= res[4 * kWaveRanks];
-          pre = (KT)((pre << d) | (KT)res[4 * kWaveRanks + 1]);
-          rem = shift;
-          __syncthreads();
+        } else if (sh1 > 0) {
+          // many keys share the top digit: 8-bit passes over the child in memory
+          int rem = sh1;
+          KT pre = top;
+          while (rem > 0) {
+            const int d = rem < 8 ? rem : 8;
+            const int shift = rem - d;
+            const int hshift = shift + d;
+            if (tid < 256) bins[tid] = 0;
+            __syncthreads();
+            each([&](KT raw) {
+              const KT rel = (KT)(to_key<C>(raw) - lo);
+              if ((KT)(rel >> hshift) == pre)
+                atomicAdd(&bins[(uint32_t)(rel >> shift) & ((1u << d) - 1u)], 1u);
+            });
+            __syncthreads();
+            if (w0) {
+              uint32_t bn, below;
+              wave_pick256(bins, lane, r, bn, below);
+              if (lane == 0) {
+                res[4 * kWaveRanks + 1] = bn;
+                res[4 * kWaveRanks] = below;
+              }
+            }
+            __syncthreads();
+            r -= res[4 * kWaveRanks];
+            pre = (KT)((pre << d) | (KT)res[4 * kWaveRanks + 1]);
+            rem = shift;
+            __syncthreads();
+          }
+          low = (KT)(pre & lowmask);
         }
-        low = (KT)(pre & lowmask);
+        if (tid == 0) out[q.out] = from_key<C>((KT)(lo + ((KT)(top << sh1) | low)));
       }
-      if (tid == 0) out[q.out] = from_key<C>((KT)(lo + ((KT)(top << sh1) | low)));
+    };
+    // from registers when the child fits (read from memory once), else from
+    // memory in every pass
+    if (GroupKeys<KT, kFRegWords>::fits(srcp, c.len)) {  // uniform
+      GroupKeys<KT, kFRegWords> gk;
+      gk.load(srcp, c.len);
+      body([&](auto&& f) { gk.for_each(f); });
+    } else {
+      body([&](auto&& f) { group_for_each(srcp, c.len, f); });
     }
     __syncthreads();
   }
