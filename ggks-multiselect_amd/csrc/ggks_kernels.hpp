@@ -592,13 +592,21 @@ __global__ __launch_bounds__(kThreads) void k_count(const KT* __restrict__ src,
   const uint4* __restrict__ pv = reinterpret_cast<const uint4*>(p + head);
   const uint64_t per_round = (uint64_t)R * kThreads;
   uint64_t r0 = 0;
+  // full rounds, software pipelined: the next round's loads are in flight
+  // while a round is bucketed and counted
   auto full_rounds = [&](auto left) {
-    for (; r0 + per_round <= nvec; r0 += per_round) {
-      uint4 v[R];
+    if (r0 + per_round > nvec) return;  // uniform
+    uint4 v[R];
 #pragma unroll
-      for (int t = 0; t < R; ++t) v[t] = stream_load(&pv[r0 + (uint64_t)t * kThreads + tid]);
+    for (int t = 0; t < R; ++t) v[t] = stream_load(&pv[r0 + (uint64_t)t * kThreads + tid]);
+    for (; r0 + per_round <= nvec; r0 += per_round) {
       KT k[N];
       unpack<KT, R>(v, k);
+      if (r0 + 2 * per_round <= nvec) {  // uniform
+#pragma unroll
+        for (int t = 0; t < R; ++t)
+          v[t] = stream_load(&pv[r0 + per_round + (uint64_t)t * kThreads + tid]);
+      }
       uint32_t b[N];
       bucket_batch1<KT, C, N, decltype(left)::value>(bk, lut, k, b);
 #pragma unroll
