@@ -1204,7 +1204,11 @@ struct GroupKeys {
     if (ht) f(tk);
   }
 };
-constexpr int kFRegWords = 10;  // 16-byte words per thread: children up to 40 KB
+// 16-byte words per thread held by k_finish_small: children up to 36 KB of
+// 4-byte keys, 32 KB of 8-byte keys (the most that fit in 128 VGPRs without
+// spilling for every key type)
+template <typename KT>
+constexpr int kFRegWords = sizeof(KT) == 4 ? 9 : 8;
 
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1416,8 +1420,8 @@ __global__ __launch_bounds__(kFThreads) __attribute__((amdgpu_waves_per_eu(4))) 
     };
     // from registers when the child fits (read from memory once), else from
     // memory in every pass
-    if (GroupKeys<KT, kFRegWords>::fits(srcp, c.len)) {  // uniform
-      GroupKeys<KT, kFRegWords> gk;
+    if (GroupKeys<KT, kFRegWords<KT>>::fits(srcp, c.len)) {  // uniform
+      GroupKeys<KT, kFRegWords<KT>> gk;
       gk.load(srcp, c.len);
       body([&](auto&& f) { gk.for_each(f); });
     } else {
