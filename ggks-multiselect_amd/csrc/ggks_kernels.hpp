@@ -383,20 +383,23 @@ __global__ void k_topk_queries(uint32_t K, uint64_t n, Query* __restrict__ q) {
   q[i].out = i;
 }
 
-// Seeded random sample of each segment: grid (segments, sample blocks of
-// 1024; every sample count is a multiple of 1024).  Each work-group also
-// writes the min and max key of its 1024 samples, so the LUT kernel reads 64
-// pairs instead of scanning the sample for its range.
+// Seeded random sample of each segment: one work-group per block of 1024
+// samples of a segment (the planner's dense table; every sample count is a
+// multiple of 1024).  Each work-group also writes the min and max key of its
+// 1024 samples, so the LUT kernel reads 64 pairs instead of scanning the
+// sample for its range.
 template <typename KT, int C>
 __global__ __launch_bounds__(kThreads) void k_sample(const KT* __restrict__ src,
                                                      const SegDesc* __restrict__ segs,
+                                                     const SampleBlock* __restrict__ sblocks,
                                                      KT* __restrict__ samples,
                                                      KT* __restrict__ minmax, uint64_t seed) {
   __shared__ KT red_lo[kThreads / 64], red_hi[kThreads / 64];
-  const SegDesc& sd = segs[blockIdx.x];
-  if (sd.mode != MODE_SAMPLE || blockIdx.y * kThreads >= sd.nsample) return;  // uniform
-  const uint32_t i = blockIdx.y * kThreads + threadIdx.x;
-  uint64_t z = seed + 0x9E3779B97F4A7C15ull * ((uint64_t)blockIdx.x * 0x100000001ull + i + 1);
+  const SampleBlock sb = sblocks[blockIdx.x];
+  const SegDesc& sd = segs[sb.seg];
+  if (sd.mode != MODE_SAMPLE || sb.blk * kThreads >= sd.nsample) return;  // uniform
+  const uint32_t i = sb.blk * kThreads + threadIdx.x;
+  uint64_t z = seed + 0x9E3779B97F4A7C15ull * ((uint64_t)sb.seg * 0x100000001ull + i + 1);
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
   z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
   z ^= z >> 31;
@@ -419,7 +422,7 @@ __global__ __launch_bounds__(kThreads) void k_sample(const KT* __restrict__ src,
       lo = red_lo[w] < lo ? red_lo[w] : lo;
       hi = red_hi[w] > hi ? red_hi[w] : hi;
     }
-    KT* mm = minmax + 2 * (sd.sample_off / kThreads + blockIdx.y);
+    KT* mm = minmax + 2 * (sd.sample_off / kThreads + sb.blk);
     mm[0] = lo;
     mm[1] = hi;
   }
@@ -629,13 +632,15 @@ __global__ __launch_bounds__(kThreads) void k_count(const KT* __restrict__ src,
   for (uint32_t i = threadIdx.x; i < st.B; i += kThreads) row[i] = hist[i];
 }
 
-// Sum the chunk rows of kReduceBins buckets; rewrite each row entry as the
-// chunk's exclusive offset inside its bucket; write the group's bucket
-// prefixes and its sum.  1024 threads = 16 groups of
-// chunks x 64 bins; each thread keeps its (at most kReduceRegs) row entries in
-// registers between the sum and the rewrite.
-constexpr uint32_t kReduceParts = kThreads / kReduceBins;  // 16
-constexpr uint32_t kReduceRegs = kMaxChunks / kReduceParts;  // chunks per part
+// Sum the chunk rows of an item's buckets; rewrite each row entry as the
+// chunk's exclusive offset inside its bucket; write the bucket prefixes
+// inside each group of kReduceBins and the group sums.  The 16 wavefronts
+// of a work-group are (chunk parts) x (2^gi_log2 groups of 64 bins): a segment with
+// many chunks splits them over 16 parts, one with few chunks covers up to 16
+// groups per work-group instead.  Each lane keeps its (at most kReduceRegs)
+// row entries in registers between the sum and the rewrite.
+constexpr uint32_t kReduceWaves = kThreads / kReduceBins;  // 16
+constexpr uint32_t kReduceRegs = kMaxChunks / kReduceWaves;  // chunks per part
 
 __global__ __launch_bounds__(kThreads) void k_reduce(const SegDesc* __restrict__ segs,
                                                      const SegState* __restrict__ states,
@@ -643,16 +648,20 @@ __global__ __launch_bounds__(kThreads) void k_reduce(const SegDesc* __restrict__
                                                      uint32_t* __restrict__ rows,
                                                      uint32_t* __restrict__ totals,
                                                      uint32_t* __restrict__ gsums) {
-  __shared__ uint32_t part[kReduceParts][kReduceBins];
+  __shared__ uint32_t part[kReduceWaves][kReduceBins];
   const ReduceItem it = items[blockIdx.x];
   const SegDesc& sd = segs[it.seg];
   const uint32_t B = states[it.seg].B;
-  const uint32_t lane = threadIdx.x % kReduceBins, q = threadIdx.x / kReduceBins;
-  const uint32_t bin = it.group * kReduceBins + lane;
+  const uint32_t gl = it.gi_log2, pl = 4 - gl;  // log2 of groups and of chunk parts
+  const uint32_t parts = 1u << pl;
+  const uint32_t lane = threadIdx.x % kReduceBins, w = threadIdx.x / kReduceBins;
+  const uint32_t g = w & ((1u << gl) - 1u), q = w >> gl;  // this wavefront's group and part
+  const uint32_t group = it.group + g;
+  const uint32_t bin = group * kReduceBins + lane;
   const bool valid = bin < B;
   const uint32_t nch = sd.nchunks;  // <= kMaxChunks (planner)
-  const uint32_t c0 = (uint32_t)((uint64_t)nch * q / kReduceParts);
-  const uint32_t c1 = (uint32_t)((uint64_t)nch * (q + 1) / kReduceParts);
+  const uint32_t c0 = (nch * q) >> pl;
+  const uint32_t c1 = (nch * (q + 1)) >> pl;
   uint32_t* base = rows + sd.row_base + bin;
   const uint64_t stride = sd.bmax;
   uint32_t v[kReduceRegs];
@@ -662,14 +671,16 @@ __global__ __launch_bounds__(kThreads) void k_reduce(const SegDesc* __restrict__
     v[u] = (valid && c0 + u < c1) ? base[(uint64_t)(c0 + u) * stride] : 0u;
     sum += v[u];
   }
-  part[q][lane] = sum;
+  part[w][lane] = sum;
   __syncthreads();
   uint32_t off = 0, total = 0;
 #pragma unroll
-  for (uint32_t j = 0; j < kReduceParts; ++j) {
-    uint32_t w = part[j][lane];
-    off += j < q ? w : 0u;
-    total += w;
+  for (uint32_t j = 0; j < kReduceWaves; ++j) {
+    if (j < parts) {
+      const uint32_t x = part[(j << gl) + g][lane];
+      off += j < q ? x : 0u;
+      total += x;
+    }
   }
   uint32_t run = off;
 #pragma unroll
@@ -677,13 +688,14 @@ __global__ __launch_bounds__(kThreads) void k_reduce(const SegDesc* __restrict__
     if (valid && c0 + u < c1) base[(uint64_t)(c0 + u) * stride] = run;
     run += v[u];
   }
-  // wavefront 0 holds the 64 bins of the item: their exclusive prefix inside
-  // the group, and the group's sum (k_findk scans the group sums only)
-  if (q == 0) {
+  // the part-0 wavefront of each group holds its 64 bins: their exclusive
+  // prefix inside the group, and the group's sum (k_findk scans the group
+  // sums only); groups past the segment's bucket budget are not written
+  if (q == 0 && group * kReduceBins < sd.bmax) {
     const uint32_t t = valid ? total : 0u;
     const uint32_t inc = wave_incl_scan(t);
     if (valid) totals[sd.tot_off + bin] = inc - t;
-    if (lane == kReduceBins - 1) gsums[sd.tot_off / kReduceBins + it.group] = inc;
+    if (lane == kReduceBins - 1) gsums[sd.tot_off / kReduceBins + group] = inc;
   }
 }
 
@@ -961,7 +973,7 @@ __global__ __launch_bounds__(kThreads) void k_compact(const KT* __restrict__ src
   uint32_t* cur = lds;
   KT* kb = reinterpret_cast<KT*>(cur + kCompactCursors);
   const uint32_t tid = threadIdx.x, lane = lane_id();
-  strided_for(lut_pool + sd.lut_off, st.P, [&](uint32_t i, uint32_t v) { lut[i] = v; });
+  strided_for(lut_pool + sd.lut_off, st.P, [&](uint32_t i, uint32_t x) { lut[i] = x; });
   for (uint32_t i = tid; i < nwords; i += kThreads) {
     const uint32_t w = bits_pool[sd.bits_off + i];
     abits[i] = w;
@@ -1260,174 +1272,191 @@ __global__ __launch_bounds__(kFThreads) __attribute__((amdgpu_waves_per_eu(4))) 
   const bool w0 = tid < 64;
   KT* buf = reinterpret_cast<KT*>(h);  // gathered keys (after the histogram is read)
   uint32_t* bins = h + kFCollectWords;  // 256 bins for the LDS radix passes
-  for (uint32_t ci = blockIdx.x; ci < nchildren; ci += gridDim.x) {
-    const Child c = children[ci];
-    if (c.q_cnt == 0 || !wave_child(c)) continue;  // uniform
-    const KT* srcp = (c.src ? red : input) + c.start;
-    // the passes over the child's keys, given a visitor of its keys
-    auto body = [&](auto&& each) {
-      KT lo = (KT)c.lo, hi = (KT)c.hi;
-      if (c.need_range) {
-        KT a = KeyBits<KT>::MAX, b = 0;
-        each([&](KT raw) {
-          const KT k = to_key<C>(raw);
-          a = k < a ? k : a;
-          b = k > b ? k : b;
-        });
-        for (int d = 32; d >= 1; d >>= 1) {
-          const KT x = __shfl_xor(a, d, 64), y = __shfl_xor(b, d, 64);
-          a = x < a ? x : a;
-          b = y > b ? y : b;
-        }
-        if (lane == 0) {
-          red_lo[tid >> 6] = a;
-          red_hi[tid >> 6] = b;
-        }
-        __syncthreads();
-        lo = red_lo[0];
-        hi = red_hi[0];
-        for (uint32_t w = 1; w < kFThreads / 64; ++w) {
-          lo = red_lo[w] < lo ? red_lo[w] : lo;
-          hi = red_hi[w] > hi ? red_hi[w] : hi;
-        }
-      }
-      const int nbits = bitlen<KT>((KT)(hi - lo));
-      if (nbits == 0) {  // every key equal
-        for (uint32_t qi = tid; qi < c.q_cnt; qi += kFThreads)
-          out[cq[c.q_off + qi].out] = from_key<C>(lo);
-        return;
-      }
-      // pass 1: histogram of the top digit
-      const int d1 = nbits < (int)kFDigit ? nbits : (int)kFDigit;
-      const int sh1 = nbits - d1;
-      for (uint32_t j = tid; j < kFWords; j += kFThreads) h[j] = 0;
-      __syncthreads();
-      each([&](KT raw) {
-        const uint32_t bn = (uint32_t)((KT)(to_key<C>(raw) - lo) >> sh1);
-        atomicAdd(&h[bn + (bn >> 5)], 1u);
-      });
-      __syncthreads();
-      if (w0) {  // each rank's top digit, residual rank and digit count
-        uint32_t lsum = 0;  // lane's bins lane*32 .. lane*32 + 31
-  #pragma unroll 8
-        for (uint32_t j = 0; j < 32; ++j) lsum += h[lane * 33 + j];
-        const uint32_t lex = wave_incl_scan(lsum) - lsum;
-        for (uint32_t qi = 0; qi < c.q_cnt; ++qi) {
-          const uint32_t r = cq[c.q_off + qi].rank - 1;
-          const int L = __ffsll((unsigned long long)__ballot(r >= lex && r < lex + lsum)) - 1;
-          const uint32_t eL = (uint32_t)__shfl((int)lex, L, 64);
-          const uint32_t v = lane < 32 ? h[(uint32_t)L * 33 + lane] : 0u;
-          const uint32_t ex = wave_incl_scan(v) - v + eL;
-          const int j = __ffsll((unsigned long long)__ballot(lane < 32 && r >= ex && r < ex + v)) - 1;
-          const uint32_t exj = (uint32_t)__shfl((int)ex, j, 64);  // every lane takes part
-          const uint32_t vj = (uint32_t)__shfl((int)v, j, 64);
+  __shared__ uint32_t list[kFThreads + 1];  // this window's children for this finisher; count
+  // this work-group's children are blockIdx.x, + gridDim.x, ...; they are
+  // screened kFThreads at a time, one per thread, and only those for this
+  // finisher are walked
+  const uint32_t ncand =
+      nchildren > blockIdx.x ? (nchildren - blockIdx.x + gridDim.x - 1) / gridDim.x : 0u;
+  for (uint32_t wstart = 0; wstart < ncand; wstart += kFThreads) {  // uniform
+    if (tid == 0) list[kFThreads] = 0;
+    __syncthreads();
+    if (wstart + tid < ncand) {
+      const uint32_t ci = blockIdx.x + (wstart + tid) * gridDim.x;
+      const Child cc = children[ci];
+      if (cc.q_cnt != 0 && wave_child(cc)) list[atomicAdd(&list[kFThreads], 1u)] = ci;
+    }
+    __syncthreads();
+    const uint32_t nl = list[kFThreads];
+    for (uint32_t li = 0; li < nl; ++li) {
+      const Child c = children[list[li]];
+      const KT* srcp = (c.src ? red : input) + c.start;
+      // the passes over the child's keys, given a visitor of its keys
+      auto body = [&](auto&& each) {
+        KT lo = (KT)c.lo, hi = (KT)c.hi;
+        if (c.need_range) {
+          KT a = KeyBits<KT>::MAX, b = 0;
+          each([&](KT raw) {
+            const KT k = to_key<C>(raw);
+            a = k < a ? k : a;
+            b = k > b ? k : b;
+          });
+          for (int d = 32; d >= 1; d >>= 1) {
+            const KT x = __shfl_xor(a, d, 64), y = __shfl_xor(b, d, 64);
+            a = x < a ? x : a;
+            b = y > b ? y : b;
+          }
           if (lane == 0) {
-            res[3 * qi] = (uint32_t)L * 32 + (uint32_t)j;
-            res[3 * qi + 1] = r - exj;
-            res[3 * qi + 2] = vj;
+            red_lo[tid >> 6] = a;
+            red_hi[tid >> 6] = b;
+          }
+          __syncthreads();
+          lo = red_lo[0];
+          hi = red_hi[0];
+          for (uint32_t w = 1; w < kFThreads / 64; ++w) {
+            lo = red_lo[w] < lo ? red_lo[w] : lo;
+            hi = red_hi[w] > hi ? red_hi[w] : hi;
           }
         }
-      }
-      __syncthreads();
-      const KT lowmask = sh1 == 0 ? KT(0) : (KT)(KeyBits<KT>::MAX >> (W - sh1));
-      for (uint32_t qi = 0; qi < c.q_cnt; ++qi) {
-        const Query q = cq[c.q_off + qi];
-        const KT top = (KT)res[3 * qi];
-        uint32_t r = res[3 * qi + 1];
-        const uint32_t n2 = res[3 * qi + 2];
-        KT low = 0;  // the rank's key minus lo, below the top digit
-        if (sh1 > 0 && n2 <= CAP) {
-          // pass 2: gather the keys with this top digit
-          if (tid == 0) res[4 * kWaveRanks] = 0;
-          __syncthreads();
-          each([&](KT raw) {
-            const KT rel = (KT)(to_key<C>(raw) - lo);
-            if ((KT)(rel >> sh1) == top) buf[atomicAdd(&res[4 * kWaveRanks], 1u)] = rel & lowmask;
-          });
-          __syncthreads();
-          if (w0) {
-            if (n2 <= 64) {
-              // rank by counting: lane i holds key i; ties broken by position
-              const KT mine = lane < n2 ? buf[lane] : KeyBits<KT>::MAX;
-              uint32_t below = 0;
-              for (uint32_t j = 0; j < n2; ++j) {
-                const KT o = buf[j];
-                below += (o < mine || (o == mine && j < lane)) ? 1u : 0u;
-              }
-              const int src = __ffsll((unsigned long long)__ballot(lane < n2 && below == r)) - 1;
-              low = (KT)__shfl(mine, src, 64);
-            } else {
-              int rem = sh1;
-              KT pre = 0;
-              while (rem > 0) {
-                const int d = rem < 8 ? rem : 8;
-                const int shift = rem - d;
-                const int hshift = shift + d;
-  #pragma unroll
-                for (int j = 0; j < 4; ++j) bins[lane * 4 + j] = 0;
-                wave_sync();
-                for (uint32_t i = lane; i < n2; i += 64) {
-                  const KT v = buf[i];
-                  const KT t = hshift >= (int)W ? KT(0) : (KT)(v >> hshift);
-                  if (t == pre) atomicAdd(&bins[(uint32_t)(v >> shift) & ((1u << d) - 1u)], 1u);
-                }
-                wave_sync();
-                uint32_t bn, below;
-                wave_pick256(bins, lane, r, bn, below);
-                r -= below;
-                pre = (KT)((pre << d) | (KT)bn);
-                rem = shift;
-                wave_sync();
-              }
-              low = pre;
+        const int nbits = bitlen<KT>((KT)(hi - lo));
+        if (nbits == 0) {  // every key equal
+          for (uint32_t qi = tid; qi < c.q_cnt; qi += kFThreads)
+            out[cq[c.q_off + qi].out] = from_key<C>(lo);
+          return;
+        }
+        // pass 1: histogram of the top digit
+        const int d1 = nbits < (int)kFDigit ? nbits : (int)kFDigit;
+        const int sh1 = nbits - d1;
+        for (uint32_t j = tid; j < kFWords; j += kFThreads) h[j] = 0;
+        __syncthreads();
+        each([&](KT raw) {
+          const uint32_t bn = (uint32_t)((KT)(to_key<C>(raw) - lo) >> sh1);
+          atomicAdd(&h[bn + (bn >> 5)], 1u);
+        });
+        __syncthreads();
+        if (w0) {  // each rank's top digit, residual rank and digit count
+          uint32_t lsum = 0;  // lane's bins lane*32 .. lane*32 + 31
+    #pragma unroll 8
+          for (uint32_t j = 0; j < 32; ++j) lsum += h[lane * 33 + j];
+          const uint32_t lex = wave_incl_scan(lsum) - lsum;
+          for (uint32_t qi = 0; qi < c.q_cnt; ++qi) {
+            const uint32_t r = cq[c.q_off + qi].rank - 1;
+            const int L = __ffsll((unsigned long long)__ballot(r >= lex && r < lex + lsum)) - 1;
+            const uint32_t eL = (uint32_t)__shfl((int)lex, L, 64);
+            const uint32_t v = lane < 32 ? h[(uint32_t)L * 33 + lane] : 0u;
+            const uint32_t ex = wave_incl_scan(v) - v + eL;
+            const int j = __ffsll((unsigned long long)__ballot(lane < 32 && r >= ex && r < ex + v)) - 1;
+            const uint32_t exj = (uint32_t)__shfl((int)ex, j, 64);  // every lane takes part
+            const uint32_t vj = (uint32_t)__shfl((int)v, j, 64);
+            if (lane == 0) {
+              res[3 * qi] = (uint32_t)L * 32 + (uint32_t)j;
+              res[3 * qi + 1] = r - exj;
+              res[3 * qi + 2] = vj;
             }
           }
-          __syncthreads();
-        } else if (sh1 > 0) {
-          // many keys share the top digit: 8-bit passes over the child in memory
-          int rem = sh1;
-          KT pre = top;
-          while (rem > 0) {
-            const int d = rem < 8 ? rem : 8;
-            const int shift = rem - d;
-            const int hshift = shift + d;
-            if (tid < 256) bins[tid] = 0;
+        }
+        __syncthreads();
+        const KT lowmask = sh1 == 0 ? KT(0) : (KT)(KeyBits<KT>::MAX >> (W - sh1));
+        for (uint32_t qi = 0; qi < c.q_cnt; ++qi) {
+          const Query q = cq[c.q_off + qi];
+          const KT top = (KT)res[3 * qi];
+          uint32_t r = res[3 * qi + 1];
+          const uint32_t n2 = res[3 * qi + 2];
+          KT low = 0;  // the rank's key minus lo, below the top digit
+          if (sh1 > 0 && n2 <= CAP) {
+            // pass 2: gather the keys with this top digit
+            if (tid == 0) res[4 * kWaveRanks] = 0;
             __syncthreads();
             each([&](KT raw) {
               const KT rel = (KT)(to_key<C>(raw) - lo);
-              if ((KT)(rel >> hshift) == pre)
-                atomicAdd(&bins[(uint32_t)(rel >> shift) & ((1u << d) - 1u)], 1u);
+              if ((KT)(rel >> sh1) == top) buf[atomicAdd(&res[4 * kWaveRanks], 1u)] = rel & lowmask;
             });
             __syncthreads();
             if (w0) {
-              uint32_t bn, below;
-              wave_pick256(bins, lane, r, bn, below);
-              if (lane == 0) {
-                res[4 * kWaveRanks + 1] = bn;
-                res[4 * kWaveRanks] = below;
+              if (n2 <= 64) {
+                // rank by counting: lane i holds key i; ties broken by position
+                const KT mine = lane < n2 ? buf[lane] : KeyBits<KT>::MAX;
+                uint32_t below = 0;
+                for (uint32_t j = 0; j < n2; ++j) {
+                  const KT o = buf[j];
+                  below += (o < mine || (o == mine && j < lane)) ? 1u : 0u;
+                }
+                const int src = __ffsll((unsigned long long)__ballot(lane < n2 && below == r)) - 1;
+                low = (KT)__shfl(mine, src, 64);
+              } else {
+                int rem = sh1;
+                KT pre = 0;
+                while (rem > 0) {
+                  const int d = rem < 8 ? rem : 8;
+                  const int shift = rem - d;
+                  const int hshift = shift + d;
+    #pragma unroll
+                  for (int j = 0; j < 4; ++j) bins[lane * 4 + j] = 0;
+                  wave_sync();
+                  for (uint32_t i = lane; i < n2; i += 64) {
+                    const KT v = buf[i];
+                    const KT t = hshift >= (int)W ? KT(0) : (KT)(v >> hshift);
+                    if (t == pre) atomicAdd(&bins[(uint32_t)(v >> shift) & ((1u << d) - 1u)], 1u);
+                  }
+                  wave_sync();
+                  uint32_t bn, below;
+                  wave_pick256(bins, lane, r, bn, below);
+                  r -= below;
+                  pre = (KT)((pre << d) | (KT)bn);
+                  rem = shift;
+                  wave_sync();
+                }
+                low = pre;
               }
             }
             __syncthreads();
-            r -= res[4 * kWaveRanks];
-            pre = (KT)((pre << d) | (KT)res[4 * kWaveRanks + 1]);
-            rem = shift;
-            __syncthreads();
+          } else if (sh1 > 0) {
+            // many keys share the top digit: 8-bit passes over the child in memory
+            int rem = sh1;
+            KT pre = top;
+            while (rem > 0) {
+              const int d = rem < 8 ? rem : 8;
+              const int shift = rem - d;
+              const int hshift = shift + d;
+              if (tid < 256) bins[tid] = 0;
+              __syncthreads();
+              each([&](KT raw) {
+                const KT rel = (KT)(to_key<C>(raw) - lo);
+                if ((KT)(rel >> hshift) == pre)
+                  atomicAdd(&bins[(uint32_t)(rel >> shift) & ((1u << d) - 1u)], 1u);
+              });
+              __syncthreads();
+              if (w0) {
+                uint32_t bn, below;
+                wave_pick256(bins, lane, r, bn, below);
+                if (lane == 0) {
+                  res[4 * kWaveRanks + 1] = bn;
+                  res[4 * kWaveRanks] = below;
+                }
+              }
+              __syncthreads();
+              r -= res[4 * kWaveRanks];
+              pre = (KT)((pre << d) | (KT)res[4 * kWaveRanks + 1]);
+              rem = shift;
+              __syncthreads();
+            }
+            low = (KT)(pre & lowmask);
           }
-          low = (KT)(pre & lowmask);
+          if (tid == 0) out[q.out] = from_key<C>((KT)(lo + ((KT)(top << sh1) | low)));
         }
-        if (tid == 0) out[q.out] = from_key<C>((KT)(lo + ((KT)(top << sh1) | low)));
+      };
+      // from registers when the child fits (read from memory once), else from
+      // memory in every pass
+      if (GroupKeys<KT, kFRegWords<KT>>::fits(srcp, c.len)) {  // uniform
+        GroupKeys<KT, kFRegWords<KT>> gk;
+        gk.load(srcp, c.len);
+        body([&](auto&& f) { gk.for_each(f); });
+      } else {
+        body([&](auto&& f) { group_for_each(srcp, c.len, f); });
       }
-    };
-    // from registers when the child fits (read from memory once), else from
-    // memory in every pass
-    if (GroupKeys<KT, kFRegWords<KT>>::fits(srcp, c.len)) {  // uniform
-      GroupKeys<KT, kFRegWords<KT>> gk;
-      gk.load(srcp, c.len);
-      body([&](auto&& f) { gk.for_each(f); });
-    } else {
-      body([&](auto&& f) { group_for_each(srcp, c.len, f); });
+      __syncthreads();
     }
-    __syncthreads();
+    __syncthreads();  // every thread is done with the list before it is refilled
   }
 }
 
@@ -1441,7 +1470,8 @@ constexpr uint32_t kFinishOffGpre = kFinishOffQpre + kQCap * 8;
 constexpr uint32_t kFinishOffRed = kFinishOffGpre + (kGroupMax + 2) * 8;
 constexpr uint32_t kFinishOffGfirst = kFinishOffRed + 32 * 8;
 constexpr uint32_t kFinishOffMisc = kFinishOffGfirst + ((kQCap + 1) * 4 + 15) / 16 * 16;
-constexpr uint32_t kFinishLdsTotal = kFinishOffMisc + 32 * 4;
+constexpr uint32_t kFinishOffList = kFinishOffMisc + 32 * 4;
+constexpr uint32_t kFinishLdsTotal = kFinishOffList + (kThreads + 4) * 4;
 template <typename KT, int C>
 __global__ __launch_bounds__(kThreads) void k_finish(const KT* __restrict__ input,
                                                      const KT* __restrict__ red,
@@ -1462,185 +1492,204 @@ __global__ __launch_bounds__(kThreads) void k_finish(const KT* __restrict__ inpu
   KT* red_hi = red_lo + 16;                                          // 16
   uint32_t* gfirst = reinterpret_cast<uint32_t*>(base + kFinishOffGfirst);  // kQCap + 1
   uint32_t* misc = reinterpret_cast<uint32_t*>(base + kFinishOffMisc);      // 32 words
+  uint32_t* list = reinterpret_cast<uint32_t*>(base + kFinishOffList);      // kThreads + count
+  uint32_t& nlist = list[kThreads];
   const uint32_t tid = threadIdx.x;
 
-  for (uint32_t ci = blockIdx.x; ci < nchildren; ci += gridDim.x) {
-    const Child c = children[ci];
-    if (c.q_cnt == 0 || wave_child(c)) continue;
-    const KT* srcp = (c.src ? red : input) + c.start;
-    const bool inl = c.len <= CAP;
-    KT lo = (KT)c.lo, hi = (KT)c.hi;
-    if (inl) {
-      for (uint32_t i = tid; i < c.len; i += kThreads) keys[i] = to_key<C>(srcp[i]);
-    }
-    if (c.need_range) {
-      KT a = KeyBits<KT>::MAX, b = 0;
-      for (uint32_t i = tid; i < c.len; i += kThreads) {
-        KT k = inl ? keys[i] : to_key<C>(srcp[i]);
-        a = k < a ? k : a;
-        b = k > b ? k : b;
-      }
-      for (int d = 32; d >= 1; d >>= 1) {
-        KT x = __shfl_xor(a, d, 64), y = __shfl_xor(b, d, 64);
-        a = x < a ? x : a;
-        b = y > b ? y : b;
-      }
-      if (lane_id() == 0) { red_lo[tid >> 6] = a; red_hi[tid >> 6] = b; }
-      __syncthreads();
-      lo = red_lo[0]; hi = red_hi[0];
-      for (int w = 1; w < 16; ++w) {
-        lo = red_lo[w] < lo ? red_lo[w] : lo;
-        hi = red_hi[w] > hi ? red_hi[w] : hi;
-      }
+  // this work-group's children are blockIdx.x, + gridDim.x, ...; they are
+  // screened kThreads at a time, one per thread, and only those for this
+  // finisher are walked (a table of mostly small children costs one round
+  // of loads per window, not one dependent load per child)
+  const uint32_t ncand =
+      nchildren > blockIdx.x ? (nchildren - blockIdx.x + gridDim.x - 1) / gridDim.x : 0u;
+  for (uint32_t w0 = 0; w0 < ncand; w0 += kThreads) {  // uniform
+    if (tid == 0) nlist = 0;
+    __syncthreads();
+    if (w0 + tid < ncand) {
+      const uint32_t ci = blockIdx.x + (w0 + tid) * gridDim.x;
+      const Child c = children[ci];
+      if (c.q_cnt != 0 && !wave_child(c)) list[atomicAdd(&nlist, 1u)] = ci;
     }
     __syncthreads();
-    if (inl) {
-      for (uint32_t i = tid; i < c.len; i += kThreads) keys[i] -= lo;
-    }
-    const int nbits = bitlen<KT>(hi - lo);
-
-    if (inl && c.q_cnt > kSortRanks) {
-      // many ranks: sort the child's keys in LDS, then every rank is a lookup
-      uint32_t np = 1;
-      while (np < c.len) np <<= 1;
-      for (uint32_t i = c.len + tid; i < np; i += kThreads) keys[i] = KeyBits<KT>::MAX;
+    const uint32_t nl = nlist;
+    for (uint32_t li = 0; li < nl; ++li) {
+      const Child c = children[list[li]];
+      const KT* srcp = (c.src ? red : input) + c.start;
+      const bool inl = c.len <= CAP;
+      KT lo = (KT)c.lo, hi = (KT)c.hi;
+      if (inl) {
+        for (uint32_t i = tid; i < c.len; i += kThreads) keys[i] = to_key<C>(srcp[i]);
+      }
+      if (c.need_range) {
+        KT a = KeyBits<KT>::MAX, b = 0;
+        for (uint32_t i = tid; i < c.len; i += kThreads) {
+          KT k = inl ? keys[i] : to_key<C>(srcp[i]);
+          a = k < a ? k : a;
+          b = k > b ? k : b;
+        }
+        for (int d = 32; d >= 1; d >>= 1) {
+          KT x = __shfl_xor(a, d, 64), y = __shfl_xor(b, d, 64);
+          a = x < a ? x : a;
+          b = y > b ? y : b;
+        }
+        if (lane_id() == 0) { red_lo[tid >> 6] = a; red_hi[tid >> 6] = b; }
+        __syncthreads();
+        lo = red_lo[0]; hi = red_hi[0];
+        for (int w = 1; w < 16; ++w) {
+          lo = red_lo[w] < lo ? red_lo[w] : lo;
+          hi = red_hi[w] > hi ? red_hi[w] : hi;
+        }
+      }
       __syncthreads();
-      for (uint32_t kk = 2; kk <= np; kk <<= 1) {
-        for (uint32_t j = kk >> 1; j > 0; j >>= 1) {
-          for (uint32_t i = tid; i < np; i += kThreads) {
-            uint32_t ix = i ^ j;
-            if (ix > i) {
-              KT a = keys[i], b = keys[ix];
-              bool up = (i & kk) == 0;
-              if ((a > b) == up) { keys[i] = b; keys[ix] = a; }
+      if (inl) {
+        for (uint32_t i = tid; i < c.len; i += kThreads) keys[i] -= lo;
+      }
+      const int nbits = bitlen<KT>(hi - lo);
+
+      if (inl && c.q_cnt > kSortRanks) {
+        // many ranks: sort the child's keys in LDS, then every rank is a lookup
+        uint32_t np = 1;
+        while (np < c.len) np <<= 1;
+        for (uint32_t i = c.len + tid; i < np; i += kThreads) keys[i] = KeyBits<KT>::MAX;
+        __syncthreads();
+        for (uint32_t kk = 2; kk <= np; kk <<= 1) {
+          for (uint32_t j = kk >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = tid; i < np; i += kThreads) {
+              uint32_t ix = i ^ j;
+              if (ix > i) {
+                KT a = keys[i], b = keys[ix];
+                bool up = (i & kk) == 0;
+                if ((a > b) == up) { keys[i] = b; keys[ix] = a; }
+              }
             }
+            __syncthreads();
           }
-          __syncthreads();
         }
-      }
-      for (uint32_t i = tid; i < c.q_cnt; i += kThreads) {
-        Query q = cq[c.q_off + i];
-        out[q.out] = from_key<C>((KT)(lo + keys[q.rank - 1]));
-      }
-      __syncthreads();
-      continue;
-    }
-
-    for (uint32_t q0 = 0; q0 < c.q_cnt; q0 += kQCap) {
-      const uint32_t nq = min(kQCap, c.q_cnt - q0);
-      // queries sorted by rank: bitonic sort of the padded power of two
-      uint32_t np = 1;
-      while (np < nq) np <<= 1;
-      for (uint32_t i = tid; i < np; i += kThreads) {
-        if (i < nq) {
-          Query q = cq[c.q_off + q0 + i];
-          qv[i] = ((uint64_t)q.rank << 32) | q.out;
-        } else {
-          qv[i] = ~0ull;
-        }
-      }
-      __syncthreads();
-      for (uint32_t kk = 2; kk <= np; kk <<= 1) {
-        for (uint32_t j = kk >> 1; j > 0; j >>= 1) {
-          for (uint32_t i = tid; i < np; i += kThreads) {
-            uint32_t ix = i ^ j;
-            if (ix > i) {
-              uint64_t a = qv[i], b = qv[ix];
-              bool up = (i & kk) == 0;
-              if ((a > b) == up) { qv[i] = b; qv[ix] = a; }
-            }
-          }
-          __syncthreads();
-        }
-      }
-      for (uint32_t i = tid; i < nq; i += kThreads) qpre[i] = 0;
-      __syncthreads();
-
-      // digit levels from the top of the relative range
-      int rem = nbits;
-      while (rem > 0) {
-        const int d = rem < 8 ? rem : 8;
-        const int shift = rem - d;
-        const int hshift = shift + d;  // bits above the digit
-        // groups = runs of equal prefix (queries are rank-sorted, so prefixes
-        // are non-decreasing)
-        if (tid == 0) {
-          uint32_t g = 0;
-          for (uint32_t i = 0; i < nq; ++i)
-            if (i == 0 || qpre[i] != qpre[i - 1]) gfirst[g++] = i;
-          gfirst[g] = nq;
-          misc[0] = g;
+        for (uint32_t i = tid; i < c.q_cnt; i += kThreads) {
+          Query q = cq[c.q_off + i];
+          out[q.out] = from_key<C>((KT)(lo + keys[q.rank - 1]));
         }
         __syncthreads();
-        const uint32_t ng = misc[0];
-        for (uint32_t g0 = 0; g0 < ng; g0 += kGroupMax) {
-          const uint32_t gn = min(kGroupMax, ng - g0);
-          for (uint32_t i = tid; i < gn * 256; i += kThreads) hist[i] = 0;
-          for (uint32_t g = tid; g < gn; g += kThreads) gpre[g] = qpre[gfirst[g0 + g]];
-          __syncthreads();
-          auto visit = [&](KT rel) {
-            KT pre = hshift >= (int)W ? KT(0) : (rel >> hshift);
-            uint32_t g;
-            if (gn == 1) {
-              if (pre != gpre[0]) return;
-              g = 0;
-            } else {
-              uint32_t a = 0, b = gn - 1;
-              while (a < b) {
-                uint32_t mid = (a + b) >> 1;
-                if (gpre[mid] < pre) a = mid + 1; else b = mid;
-              }
-              if (gpre[a] != pre) return;
-              g = a;
-            }
-            atomicAdd(&hist[g * 256 + (uint32_t)((rel >> shift) & (KT)((1u << d) - 1))], 1u);
-          };
-          if (inl) {
-            for (uint32_t i = tid; i < c.len; i += kThreads) visit(keys[i]);
-          } else {
-            for (uint64_t i = tid; i < c.len; i += kThreads) visit(to_key<C>(srcp[i]) - lo);
-          }
-          __syncthreads();
-          // exclusive scan of each group's 256 bins: one wave per group
-          const uint32_t wv = tid >> 6, lane = lane_id();
-          for (uint32_t g = wv; g < gn; g += kThreads / 64) {
-            uint32_t* h = hist + g * 256;
-            uint32_t v0 = h[lane * 4], v1 = h[lane * 4 + 1], v2 = h[lane * 4 + 2], v3 = h[lane * 4 + 3];
-            uint32_t s = v0 + v1 + v2 + v3;
-            uint32_t e = wave_incl_scan(s) - s;
-            h[lane * 4] = e;
-            h[lane * 4 + 1] = e + v0;
-            h[lane * 4 + 2] = e + v0 + v1;
-            h[lane * 4 + 3] = e + v0 + v1 + v2;
-          }
-          __syncthreads();
-          // every query of these groups picks its digit
-          const uint32_t qa = gfirst[g0], qz = gfirst[g0 + gn];
-          for (uint32_t i = qa + tid; i < qz; i += kThreads) {
-            uint32_t a = 0, b = gn - 1;  // group of query i
-            while (a < b) {
-              uint32_t mid = (a + b + 1) >> 1;
-              if (gfirst[g0 + mid] <= i) a = mid; else b = mid - 1;
-            }
-            const uint32_t* h = hist + a * 256;
-            uint32_t r1 = (uint32_t)(qv[i] >> 32) - 1;
-            uint32_t lo_d = 0, hi_d = (1u << d) - 1;  // largest digit with h <= r1
-            while (lo_d < hi_d) {
-              uint32_t mid = (lo_d + hi_d + 1) >> 1;
-              if (h[mid] <= r1) lo_d = mid; else hi_d = mid - 1;
-            }
-            uint32_t nr = r1 - h[lo_d] + 1;
-            qv[i] = ((uint64_t)nr << 32) | (uint32_t)qv[i];
-            qpre[i] = (qpre[i] << d) | (KT)lo_d;
-          }
-          __syncthreads();
-        }
-        rem = shift;
+        continue;
       }
-      for (uint32_t i = tid; i < nq; i += kThreads)
-        out[(uint32_t)qv[i]] = from_key<C>((KT)(lo + qpre[i]));
-      __syncthreads();
+
+      for (uint32_t q0 = 0; q0 < c.q_cnt; q0 += kQCap) {
+        const uint32_t nq = min(kQCap, c.q_cnt - q0);
+        // queries sorted by rank: bitonic sort of the padded power of two
+        uint32_t np = 1;
+        while (np < nq) np <<= 1;
+        for (uint32_t i = tid; i < np; i += kThreads) {
+          if (i < nq) {
+            Query q = cq[c.q_off + q0 + i];
+            qv[i] = ((uint64_t)q.rank << 32) | q.out;
+          } else {
+            qv[i] = ~0ull;
+          }
+        }
+        __syncthreads();
+        for (uint32_t kk = 2; kk <= np; kk <<= 1) {
+          for (uint32_t j = kk >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = tid; i < np; i += kThreads) {
+              uint32_t ix = i ^ j;
+              if (ix > i) {
+                uint64_t a = qv[i], b = qv[ix];
+                bool up = (i & kk) == 0;
+                if ((a > b) == up) { qv[i] = b; qv[ix] = a; }
+              }
+            }
+            __syncthreads();
+          }
+        }
+        for (uint32_t i = tid; i < nq; i += kThreads) qpre[i] = 0;
+        __syncthreads();
+
+        // digit levels from the top of the relative range
+        int rem = nbits;
+        while (rem > 0) {
+          const int d = rem < 8 ? rem : 8;
+          const int shift = rem - d;
+          const int hshift = shift + d;  // bits above the digit
+          // groups = runs of equal prefix (queries are rank-sorted, so prefixes
+          // are non-decreasing)
+          if (tid == 0) {
+            uint32_t g = 0;
+            for (uint32_t i = 0; i < nq; ++i)
+              if (i == 0 || qpre[i] != qpre[i - 1]) gfirst[g++] = i;
+            gfirst[g] = nq;
+            misc[0] = g;
+          }
+          __syncthreads();
+          const uint32_t ng = misc[0];
+          for (uint32_t g0 = 0; g0 < ng; g0 += kGroupMax) {
+            const uint32_t gn = min(kGroupMax, ng - g0);
+            for (uint32_t i = tid; i < gn * 256; i += kThreads) hist[i] = 0;
+            for (uint32_t g = tid; g < gn; g += kThreads) gpre[g] = qpre[gfirst[g0 + g]];
+            __syncthreads();
+            auto visit = [&](KT rel) {
+              KT pre = hshift >= (int)W ? KT(0) : (rel >> hshift);
+              uint32_t g;
+              if (gn == 1) {
+                if (pre != gpre[0]) return;
+                g = 0;
+              } else {
+                uint32_t a = 0, b = gn - 1;
+                while (a < b) {
+                  uint32_t mid = (a + b) >> 1;
+                  if (gpre[mid] < pre) a = mid + 1; else b = mid;
+                }
+                if (gpre[a] != pre) return;
+                g = a;
+              }
+              atomicAdd(&hist[g * 256 + (uint32_t)((rel >> shift) & (KT)((1u << d) - 1))], 1u);
+            };
+            if (inl) {
+              for (uint32_t i = tid; i < c.len; i += kThreads) visit(keys[i]);
+            } else {
+              for (uint64_t i = tid; i < c.len; i += kThreads) visit(to_key<C>(srcp[i]) - lo);
+            }
+            __syncthreads();
+            // exclusive scan of each group's 256 bins: one wave per group
+            const uint32_t wv = tid >> 6, lane = lane_id();
+            for (uint32_t g = wv; g < gn; g += kThreads / 64) {
+              uint32_t* h = hist + g * 256;
+              uint32_t v0 = h[lane * 4], v1 = h[lane * 4 + 1], v2 = h[lane * 4 + 2], v3 = h[lane * 4 + 3];
+              uint32_t s = v0 + v1 + v2 + v3;
+              uint32_t e = wave_incl_scan(s) - s;
+              h[lane * 4] = e;
+              h[lane * 4 + 1] = e + v0;
+              h[lane * 4 + 2] = e + v0 + v1;
+              h[lane * 4 + 3] = e + v0 + v1 + v2;
+            }
+            __syncthreads();
+            // every query of these groups picks its digit
+            const uint32_t qa = gfirst[g0], qz = gfirst[g0 + gn];
+            for (uint32_t i = qa + tid; i < qz; i += kThreads) {
+              uint32_t a = 0, b = gn - 1;  // group of query i
+              while (a < b) {
+                uint32_t mid = (a + b + 1) >> 1;
+                if (gfirst[g0 + mid] <= i) a = mid; else b = mid - 1;
+              }
+              const uint32_t* h = hist + a * 256;
+              uint32_t r1 = (uint32_t)(qv[i] >> 32) - 1;
+              uint32_t lo_d = 0, hi_d = (1u << d) - 1;  // largest digit with h <= r1
+              while (lo_d < hi_d) {
+                uint32_t mid = (lo_d + hi_d + 1) >> 1;
+                if (h[mid] <= r1) lo_d = mid; else hi_d = mid - 1;
+              }
+              uint32_t nr = r1 - h[lo_d] + 1;
+              qv[i] = ((uint64_t)nr << 32) | (uint32_t)qv[i];
+              qpre[i] = (qpre[i] << d) | (KT)lo_d;
+            }
+            __syncthreads();
+          }
+          rem = shift;
+        }
+        for (uint32_t i = tid; i < nq; i += kThreads)
+          out[(uint32_t)qv[i]] = from_key<C>((KT)(lo + qpre[i]));
+        __syncthreads();
+      }
     }
+    __syncthreads();  // every thread is done with the list before it is refilled
   }
 }
 

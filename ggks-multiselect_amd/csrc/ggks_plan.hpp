@@ -39,14 +39,12 @@ struct Plan {
   std::vector<ReduceItem> items;
   std::vector<Child> direct_children;
   bool direct = false;  // single-vector plan answered by the finisher alone
-  bool any_sample = false;
   uint32_t nq = 0;
-  uint32_t max_nsample = 0;
   uint32_t bmax_max = 0;
   uint32_t pbits_max = 0;
   uint64_t seed = 0;
   // arena offsets (bytes)
-  size_t off_tables = 0, off_segs = 0, off_chunks = 0, off_items = 0, tables_end = 0;
+  size_t off_tables = 0, off_segs = 0, off_chunks = 0, off_items = 0, off_sblocks = 0, tables_end = 0;
   size_t off_states = 0, off_samples = 0, off_minmax = 0, off_lut = 0, off_rows = 0, off_tot = 0;
   size_t off_bits = 0, off_wbase = 0, off_q = 0, off_qb = 0, off_qr = 0;
   size_t off_gsum = 0;  // per-group bucket sums (k_reduce -> k_findk)
@@ -54,6 +52,7 @@ struct Plan {
   size_t off_kvals = 0, off_dchildren = 0, off_dqueries = 0;  // batch / direct tables
   std::vector<uint32_t> kvals;         // k lists of the engine segments, by q_off
   std::vector<Query> direct_queries;   // explicit queries of the direct children
+  std::vector<SampleBlock> sblocks;    // the sample kernel's work-groups, densely
 
   size_t table_bytes() const { return tables_end - off_tables; }
 
@@ -63,6 +62,8 @@ struct Plan {
       memcpy(dst + (off_chunks - off_tables), chunks.data(), chunks.size() * sizeof(Chunk));
     if (!items.empty())
       memcpy(dst + (off_items - off_tables), items.data(), items.size() * sizeof(ReduceItem));
+    if (!sblocks.empty())
+      memcpy(dst + (off_sblocks - off_tables), sblocks.data(), sblocks.size() * sizeof(SampleBlock));
     if (!kvals.empty())
       memcpy(dst + (off_kvals - off_tables), kvals.data(), kvals.size() * 4);
     if (!direct_children.empty())
@@ -159,16 +160,19 @@ inline void plan_single(Plan& p, int kind, int dtype, uint64_t n, uint32_t nq,
     c.len = std::min<uint64_t>(csz, n - i * csz);
     p.chunks.push_back(c);
   }
-  for (uint32_t g = 0; g < (bmax + kReduceBins - 1) / kReduceBins; ++g) p.items.push_back({0u, g});
-  p.any_sample = mode == MODE_SAMPLE;
-  p.max_nsample = ns;
+  {
+    const uint32_t gl = reduce_groups_log2(G), ng = (bmax + kReduceBins - 1) / kReduceBins;
+    for (uint32_t g = 0; g < ng; g += 1u << gl) p.items.push_back({0u, (uint16_t)g, (uint16_t)gl});
+  }
   p.bmax_max = bmax;
   p.pbits_max = pb;
+  for (uint32_t y = 0; y < ns / kThreads; ++y) p.sblocks.push_back({0u, y});
 
   p.off_tables = off;
   p.off_segs = take(p.segs.size() * sizeof(SegDesc));
   p.off_chunks = take(p.chunks.size() * sizeof(Chunk));
   p.off_items = take(p.items.size() * sizeof(ReduceItem));
+  p.off_sblocks = take(p.sblocks.size() * sizeof(SampleBlock));
   p.tables_end = off;
   p.off_states = take(p.segs.size() * sizeof(SegState));
   p.off_samples = take((size_t)std::max<uint32_t>(ns, 1) * eb);
@@ -301,9 +305,10 @@ inline void plan_batch(Plan& p, int dtype, const std::vector<BatchSeg>& in, cons
     csz = (csz + 63) / 64 * 64;
     G = (n + csz - 1) / csz;
     uint32_t ns = 0;
-    if (mode == MODE_SAMPLE) {
+    if (mode == MODE_SAMPLE) {  // one sample per 256 keys, 1024 to 65536
       uint64_t want = (n / 256 + 1023) / 1024 * 1024;
-      ns = (uint32_t)std::min<uint64_t>(65536, std::max<uint64_t>(4096, want));
+      ns = (uint32_t)std::min<uint64_t>(65536, std::max<uint64_t>(1024, want));
+      for (uint32_t y = 0; y < ns / kThreads; ++y) p.sblocks.push_back({(uint32_t)p.segs.size(), y});
     }
     SegDesc s = {};
     s.data_off = g.offset;
@@ -334,12 +339,12 @@ inline void plan_batch(Plan& p, int dtype, const std::vector<BatchSeg>& in, cons
       ch.len = std::min<uint64_t>(csz, n - c * csz);
       p.chunks.push_back(ch);
     }
-    for (uint32_t gi = 0; gi < (bmax + kReduceBins - 1) / kReduceBins; ++gi)
-      p.items.push_back({si, gi});
+    {
+      const uint32_t gl = reduce_groups_log2(G), ng = (bmax + kReduceBins - 1) / kReduceBins;
+      for (uint32_t gi = 0; gi < ng; gi += 1u << gl) p.items.push_back({si, (uint16_t)gi, (uint16_t)gl});
+    }
     p.kvals.resize(qoff + g.num_k, 0u);
     if (!topk) memcpy(p.kvals.data() + qoff, g.k, (size_t)g.num_k * 4);
-    p.any_sample |= mode == MODE_SAMPLE;
-    p.max_nsample = std::max(p.max_nsample, ns);
     p.bmax_max = std::max(p.bmax_max, bmax);
     p.pbits_max = std::max(p.pbits_max, pb);
     red += n;
@@ -362,6 +367,7 @@ inline void plan_batch(Plan& p, int dtype, const std::vector<BatchSeg>& in, cons
   p.off_segs = take(nseg * sizeof(SegDesc));
   p.off_chunks = take(p.chunks.size() * sizeof(Chunk));
   p.off_items = take(p.items.size() * sizeof(ReduceItem));
+  p.off_sblocks = take(p.sblocks.size() * sizeof(SampleBlock));
   p.off_kvals = take(p.kvals.size() * 4);
   p.off_dchildren = take(p.direct_children.size() * sizeof(Child));
   p.off_dqueries = take(p.direct_queries.size() * sizeof(Query));

@@ -218,10 +218,10 @@ static int enqueue_level(const Exec& x, const Plan& p, const KT* d_src, KT* d_ou
     const uint32_t nchunks = (uint32_t)p.chunks.size();
 
     stage(x, 1);
-    if (p.any_sample) {
-      dim3 g(nseg, (p.max_nsample + kThreads - 1) / kThreads);
-      hipLaunchKernelGGL((k_sample<KT, C>), g, dim3(kThreads), 0, s, d_src, segs, samples, minmax,
-                         p.seed);
+    if (!p.sblocks.empty()) {
+      const SampleBlock* sblocks = reinterpret_cast<const SampleBlock*>(A + p.off_sblocks);
+      hipLaunchKernelGGL((k_sample<KT, C>), dim3((uint32_t)p.sblocks.size()), dim3(kThreads), 0, s,
+                         d_src, segs, sblocks, samples, minmax, p.seed);
     }
     stage(x, 2);
     hipLaunchKernelGGL((k_lut<KT>), dim3(nseg), dim3(kThreads), 0, s, segs, samples, minmax, states,

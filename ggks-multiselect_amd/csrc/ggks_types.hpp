@@ -91,10 +91,27 @@ struct Chunk {            // one work-group of the streaming passes
   uint64_t len;
 };
 
-struct ReduceItem {       // kReduceBins consecutive bins of one segment
+struct SampleBlock {      // one work-group of the sample kernel: kThreads samples
   uint32_t seg;
-  uint32_t group;
+  uint32_t blk;           // which kThreads-sample block of the segment
 };
+
+struct ReduceItem {       // 2^gi_log2 groups of kReduceBins consecutive bins of one segment
+  uint32_t seg;
+  uint16_t group;         // first group
+  uint16_t gi_log2;       // groups in the item, log2: the chunk range splits 16 >> gi_log2 ways
+};
+
+// log2 of the groups per reduce item for a segment of g chunks: the 16
+// wavefronts of a work-group split into parts of the chunk range times
+// groups, with as few parts as keep a lane at <= 16 chunks (a segment of
+// <= 16 chunks covers 16 groups, 1024 buckets, per work-group; 256 chunks
+// split 16 ways).
+inline uint32_t reduce_groups_log2(uint64_t g) {
+  uint32_t parts_log2 = 0;
+  while (parts_log2 < 4 && (16ull << parts_log2) < g) ++parts_log2;
+  return 4 - parts_log2;
+}
 
 struct Child {            // one bucket to finish, or one small whole segment
   uint64_t start;         // element offset in its source buffer
