@@ -64,6 +64,19 @@ def test_batch_matches_oracle(pkg, O, dtype, nseg, lo, hi):
     check_outputs(b(d), expected(O, x, table, spec["k"]), table)
 
 
+def test_batch_many_children(pkg, O):
+    # more than 256 x 1024 queries: every finisher work-group screens its
+    # share of the child table in several windows
+    W = pkg.workloads
+    spec = W.batch_spec(4242, 1500, 10, 14, max_k=64, max_topk=4096, topk_frac=0.9)
+    table = pkg.segment_table(spec["lengths"], spec["kinds"], spec["num_k"])
+    assert int(spec["num_k"].sum()) > 256 * 1024
+    x = host_values(W, np.float32, spec)
+    d = pkg.DeviceArray.from_host(x)
+    b = pkg.Batch(np.float32, table, spec["k"])
+    check_outputs(b(d), expected(O, x, table, spec["k"]), table)
+
+
 def test_batch_graph_replays(pkg, O):
     spec, table, x = make(pkg, np.float32, 11, 200, 8, 21)
     d = pkg.DeviceArray.from_host(x)
