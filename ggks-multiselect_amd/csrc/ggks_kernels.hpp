@@ -922,42 +922,56 @@ __global__ __launch_bounds__(kThreads) void k_findk(const SegDesc* __restrict__ 
 // key is bucketed exactly as in pass 1 (LUT in static LDS) and tested in a
 // byte-per-bucket copy of the active-bucket bitmap (one LDS byte load, no bit
 // arithmetic); the active keys of a round are appended straight from
-// registers to a work-group buffer in LDS (one LDS atomic per wavefront).  The main loop issues no global stores, so its waits cover
-// only its own loads.  After each round the work-group synchronises; when
-// the buffer could not take another round it is flushed: every buffered key
-// is placed (child, LDS cursor, store) in one burst, so the keys of a child's
-// slice are written close together in time.
-constexpr uint32_t kWaves = kThreads / 64;
-constexpr int kRound = 4;                     // 16-byte loads per lane per round
-constexpr uint32_t kCompactCursors = 1024;    // LDS cursors; children beyond use global atomics
-constexpr uint32_t kCompactBufBytes = 96 * 1024;  // buffered active keys
+// registers to a work-group buffer in LDS (one LDS atomic per wavefront).
+// The main loop issues no global stores, so its waits cover only its own
+// loads.  After each round the work-group synchronises; when the buffer
+// could not take another round it is flushed: every buffered key is placed
+// (child, LDS cursor, store) in one burst, so the keys of a child's slice
+// are written close together in time.
+constexpr uint32_t kCompactCursors = 1024;  // LDS cursors; children beyond use global atomics
 
-template <typename KT>
-__host__ __device__ constexpr size_t compact_lds_bytes(uint32_t pbits, uint32_t bmax) {
-  // dynamic part: cursors and the key buffer (the LUT and bitmaps are static)
-  (void)pbits;
-  (void)bmax;
-  return (size_t)kCompactCursors * 4 + kCompactBufBytes;
+// Table sizes of k_compact.  The full instance serves any plan.  The small
+// one serves plans of at most 4096 buckets and 2^10 LUT prefixes per segment
+// in under 62 KB of LDS, so two work-groups share a CU and one chunk's table
+// loads and last flush hide behind another chunk's streaming (batches of
+// many short segments, whose chunks are 2^14..2^17 keys).
+template <bool SMALL>
+struct CompactCfg {
+  static constexpr uint32_t lut_words = SMALL ? (1u << 10) : (1u << kMaxPrefixBits);
+  static constexpr uint32_t buckets = SMALL ? 4096u : kMaxBuckets;
+  static constexpr int round = SMALL ? 2 : 4;  // 16-byte loads per lane per round
+  static constexpr uint32_t buf_bytes = SMALL ? 48u * 1024 : 96u * 1024;  // buffered active keys
+};
+inline bool compact_small(uint32_t pbits_max, uint32_t bmax_max) {
+  return (1u << pbits_max) <= CompactCfg<true>::lut_words && bmax_max <= CompactCfg<true>::buckets;
 }
 
-template <typename KT, int C>
-__global__ __launch_bounds__(kThreads) void k_compact(const KT* __restrict__ src,
-                                                      const SegDesc* __restrict__ segs,
-                                                      const SegState* __restrict__ states,
-                                                      const Chunk* __restrict__ chunks,
-                                                      const uint32_t* __restrict__ lut_pool,
-                                                      uint32_t* __restrict__ rows,
-                                                      const uint32_t* __restrict__ bits_pool,
-                                                      const uint32_t* __restrict__ wbase_pool,
-                                                      const Child* __restrict__ children,
-                                                      KT* __restrict__ red) {
+template <bool SMALL>
+__host__ __device__ constexpr size_t compact_lds_bytes() {
+  // dynamic part: cursors and the key buffer (the LUT and bitmaps are static)
+  return (size_t)kCompactCursors * 4 + CompactCfg<SMALL>::buf_bytes;
+}
+
+template <typename KT, int C, bool SMALL>
+__device__ __forceinline__ void compact_chunk(const KT* __restrict__ src,
+                                              const SegDesc* __restrict__ segs,
+                                              const SegState* __restrict__ states,
+                                              const Chunk* __restrict__ chunks,
+                                              const uint32_t* __restrict__ lut_pool,
+                                              uint32_t* __restrict__ rows,
+                                              const uint32_t* __restrict__ bits_pool,
+                                              const uint32_t* __restrict__ wbase_pool,
+                                              const Child* __restrict__ children,
+                                              KT* __restrict__ red) {
+  using Cfg = CompactCfg<SMALL>;
+  constexpr int kRound = Cfg::round;
   extern __shared__ __align__(16) uint32_t lds[];
   // static LDS at fixed addresses: no base add in the per-key address math
-  __shared__ uint32_t lut[1u << kMaxPrefixBits];
-  __shared__ uint32_t abits[kMaxBuckets / 32];  // active buckets
-  __shared__ uint32_t wbase[kMaxBuckets / 32];  // first child of each bitmap word
-  __shared__ uint32_t abyte[kMaxBuckets / 4];   // active buckets, a byte (0 or 1) each
-  __shared__ uint32_t nbuf;                     // keys in the buffer
+  __shared__ uint32_t lut[Cfg::lut_words];
+  __shared__ uint32_t abits[Cfg::buckets / 32];  // active buckets
+  __shared__ uint32_t wbase[Cfg::buckets / 32];  // first child of each bitmap word
+  __shared__ uint32_t abyte[Cfg::buckets / 4];   // active buckets, a byte (0 or 1) each
+  __shared__ uint32_t nbuf;                      // keys in the buffer
   const Chunk ch = chunks[blockIdx.x];
   const SegDesc& sd = segs[ch.seg];
   const SegState st = states[ch.seg];
@@ -965,7 +979,7 @@ __global__ __launch_bounds__(kThreads) void k_compact(const KT* __restrict__ src
   if (U == 0) return;
   constexpr uint32_t VEC = 16 / sizeof(KT);
   constexpr int N = kRound * VEC;  // keys per lane per round
-  constexpr uint32_t CAP = kCompactBufBytes / sizeof(KT);
+  constexpr uint32_t CAP = Cfg::buf_bytes / sizeof(KT);
   constexpr uint32_t ROUND_MAX = kThreads * N;  // most keys one round can append
   // a round starts with at most CAP - ROUND_MAX keys buffered, so it fits
   static_assert(CAP > ROUND_MAX, "buffer must hold more than a round");
@@ -1009,7 +1023,7 @@ __global__ __launch_bounds__(kThreads) void k_compact(const KT* __restrict__ src
   // Buffered keys are active by construction; F of them per thread go
   // through each step together so their LDS round trips overlap.
   auto flush = [&](uint32_t n) {
-    constexpr int F = 8;
+    constexpr int F = SMALL ? 4 : 8;
     uint32_t i = tid;
     for (; i + (F - 1) * kThreads < n; i += F * kThreads) {
       KT kk[F];
@@ -1109,6 +1123,26 @@ __global__ __launch_bounds__(kThreads) void k_compact(const KT* __restrict__ src
   __syncthreads();
   flush(nb);
 }
+
+#define GGKS_COMPACT_PARAMS                                                                   \
+  const KT *__restrict__ src, const SegDesc *__restrict__ segs,                               \
+      const SegState *__restrict__ states, const Chunk *__restrict__ chunks,                  \
+      const uint32_t *__restrict__ lut_pool, uint32_t *__restrict__ rows,                     \
+      const uint32_t *__restrict__ bits_pool, const uint32_t *__restrict__ wbase_pool,        \
+      const Child *__restrict__ children, KT *__restrict__ red
+template <typename KT, int C>
+__global__ __launch_bounds__(kThreads) void k_compact(GGKS_COMPACT_PARAMS) {
+  compact_chunk<KT, C, false>(src, segs, states, chunks, lut_pool, rows, bits_pool, wbase_pool,
+                              children, red);
+}
+// at most 64 VGPRs: two 1024-thread work-groups per CU
+template <typename KT, int C>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) void k_compact_small(
+    GGKS_COMPACT_PARAMS) {
+  compact_chunk<KT, C, true>(src, segs, states, chunks, lut_pool, rows, bits_pool, wbase_pool,
+                             children, red);
+}
+#undef GGKS_COMPACT_PARAMS
 
 // Plain streaming copy of nv 16-byte words (grid-stride).
 __global__ __launch_bounds__(kThreads) void k_copy16(const uint4* __restrict__ src,

@@ -168,7 +168,8 @@ static void level_attrs(const Plan&) {
     set_lds_limit(k_count<KT, C>, ((size_t)kMaxBuckets + (1u << kMaxPrefixBits)) * 4);
     set_lds_limit(k_findk<KT, C>, ((size_t)kMaxBuckets + (1u << kMaxPrefixBits) +
                                    3 * ((kMaxBuckets + 31) / 32) + 32) * 4);
-    set_lds_limit(k_compact<KT, C>, compact_lds_bytes<KT>(kMaxPrefixBits, kMaxBuckets));
+    set_lds_limit(k_compact<KT, C>, compact_lds_bytes<false>());
+    set_lds_limit(k_compact_small<KT, C>, compact_lds_bytes<true>());
     set_lds_limit(k_finish<KT, C>, kFinishLdsTotal);
     return true;
   }();
@@ -240,9 +241,14 @@ static int enqueue_level(const Exec& x, const Plan& p, const KT* d_src, KT* d_ou
                        s, segs, states, lut, tot, gsum, q, d_k, x.status, qb, qr, kids, cq, bits, wbase,
                        d_out);
     stage(x, 6);
-    hipLaunchKernelGGL((k_compact<KT, C>), dim3(nchunks), dim3(kThreads),
-                       compact_lds_bytes<KT>(p.pbits_max, p.bmax_max), s, d_src, segs, states,
-                       chunks, lut, rows, bits, wbase, kids, red);
+    if (compact_small(p.pbits_max, p.bmax_max))
+      hipLaunchKernelGGL((k_compact_small<KT, C>), dim3(nchunks), dim3(kThreads),
+                         compact_lds_bytes<true>(), s, d_src, segs, states, chunks, lut, rows,
+                         bits, wbase, kids, red);
+    else
+      hipLaunchKernelGGL((k_compact<KT, C>), dim3(nchunks), dim3(kThreads),
+                         compact_lds_bytes<false>(), s, d_src, segs, states, chunks, lut, rows,
+                         bits, wbase, kids, red);
     stage(x, 7);
     finish<KT, C>(s, d_src, red, kids, p.nq, cq, d_out, 256);
   } else {
