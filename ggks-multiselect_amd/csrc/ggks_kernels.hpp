@@ -385,28 +385,40 @@ __global__ void k_topk_queries(uint32_t K, uint64_t n, Query* __restrict__ q) {
 
 // Seeded random sample of each segment: one work-group per block of 1024
 // samples of a segment (the planner's dense table; every sample count is a
-// multiple of 1024).  Each work-group also writes the min and max key of its
-// 1024 samples, so the LUT kernel reads 64 pairs instead of scanning the
-// sample for its range.
+// multiple of 1024), kSampleThreads threads of kSamplePer independent
+// gathers each.  Each work-group also writes the min and max key of its 1024
+// samples, so the LUT kernel reads 64 pairs instead of scanning the sample
+// for its range.
+constexpr uint32_t kSampleThreads = 256;
+constexpr uint32_t kSamplePer = kThreads / kSampleThreads;
 template <typename KT, int C>
-__global__ __launch_bounds__(kThreads) void k_sample(const KT* __restrict__ src,
-                                                     const SegDesc* __restrict__ segs,
-                                                     const SampleBlock* __restrict__ sblocks,
-                                                     KT* __restrict__ samples,
-                                                     KT* __restrict__ minmax, uint64_t seed) {
-  __shared__ KT red_lo[kThreads / 64], red_hi[kThreads / 64];
+__global__ __launch_bounds__(kSampleThreads) void k_sample(const KT* __restrict__ src,
+                                                           const SegDesc* __restrict__ segs,
+                                                           const SampleBlock* __restrict__ sblocks,
+                                                           KT* __restrict__ samples,
+                                                           KT* __restrict__ minmax, uint64_t seed) {
+  __shared__ KT red_lo[kSampleThreads / 64], red_hi[kSampleThreads / 64];
   const SampleBlock sb = sblocks[blockIdx.x];
   const SegDesc& sd = segs[sb.seg];
   if (sd.mode != MODE_SAMPLE || sb.blk * kThreads >= sd.nsample) return;  // uniform
-  const uint32_t i = sb.blk * kThreads + threadIdx.x;
-  uint64_t z = seed + 0x9E3779B97F4A7C15ull * ((uint64_t)sb.seg * 0x100000001ull + i + 1);
-  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-  z ^= z >> 31;
-  const uint64_t pos = __umul64hi(z, sd.len);
-  const KT k = to_key<C>(src[sd.data_off + pos]);
-  samples[sd.sample_off + i] = k;
-  KT lo = k, hi = k;
+  KT k[kSamplePer];
+#pragma unroll
+  for (uint32_t j = 0; j < kSamplePer; ++j) {  // sample i of the segment
+    const uint32_t i = sb.blk * kThreads + j * kSampleThreads + threadIdx.x;
+    uint64_t z = seed + 0x9E3779B97F4A7C15ull * ((uint64_t)sb.seg * 0x100000001ull + i + 1);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    k[j] = src[sd.data_off + __umul64hi(z, sd.len)];
+  }
+  KT lo = KeyBits<KT>::MAX, hi = 0;
+#pragma unroll
+  for (uint32_t j = 0; j < kSamplePer; ++j) {
+    const KT x = to_key<C>(k[j]);
+    samples[sd.sample_off + sb.blk * kThreads + j * kSampleThreads + threadIdx.x] = x;
+    lo = x < lo ? x : lo;
+    hi = x > hi ? x : hi;
+  }
   for (int d = 32; d >= 1; d >>= 1) {
     const KT a = __shfl_xor(lo, d, 64), b = __shfl_xor(hi, d, 64);
     lo = a < lo ? a : lo;
@@ -418,7 +430,7 @@ __global__ __launch_bounds__(kThreads) void k_sample(const KT* __restrict__ src,
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    for (uint32_t w = 1; w < kThreads / 64; ++w) {
+    for (uint32_t w = 1; w < kSampleThreads / 64; ++w) {
       lo = red_lo[w] < lo ? red_lo[w] : lo;
       hi = red_hi[w] > hi ? red_hi[w] : hi;
     }
@@ -729,23 +741,26 @@ __device__ __forceinline__ uint32_t seg_out(const SegDesc& sd, const Query* __re
 }
 
 // Per segment: ranks -> buckets, children, the child map for the compaction
-// pass, and the ranks grouped by child.
-template <typename KT, int C>
-__global__ __launch_bounds__(kThreads) void k_findk(const SegDesc* __restrict__ segs,
-                                                    SegState* __restrict__ states,
-                                                    const uint32_t* __restrict__ lut_pool,
-                                                    const uint32_t* __restrict__ totals,
-                                                    const uint32_t* __restrict__ gsums,
-                                                    const Query* __restrict__ queries,
-                                                    const uint32_t* __restrict__ kraw,
-                                                    uint32_t* __restrict__ status,
-                                                    uint32_t* __restrict__ qb,
-                                                    uint32_t* __restrict__ qr,
-                                                    Child* __restrict__ children,
-                                                    Query* __restrict__ cq,
-                                                    uint32_t* __restrict__ bits_pool,
-                                                    uint32_t* __restrict__ wbase_pool,
-                                                    KT* __restrict__ out) {
+// pass, and the ranks grouped by child.  TV is the number of 16-byte words
+// of bucket prefixes a thread holds: the full instance takes up to
+// kMaxBuckets buckets, the small one up to 4096 in at most 64 VGPRs, so that
+// two work-groups share a CU (batches of many segments).
+template <typename KT, int C, uint32_t TV>
+__device__ __forceinline__ void findk_segment(const SegDesc* __restrict__ segs,
+                                              SegState* __restrict__ states,
+                                              const uint32_t* __restrict__ lut_pool,
+                                              const uint32_t* __restrict__ totals,
+                                              const uint32_t* __restrict__ gsums,
+                                              const Query* __restrict__ queries,
+                                              const uint32_t* __restrict__ kraw,
+                                              uint32_t* __restrict__ status,
+                                              uint32_t* __restrict__ qb,
+                                              uint32_t* __restrict__ qr,
+                                              Child* __restrict__ children,
+                                              Query* __restrict__ cq,
+                                              uint32_t* __restrict__ bits_pool,
+                                              uint32_t* __restrict__ wbase_pool,
+                                              KT* __restrict__ out) {
   extern __shared__ __align__(16) uint32_t lds[];
   const SegDesc sd = segs[blockIdx.x];
   // the first k of this thread, read now so its latency hides under the
@@ -766,7 +781,6 @@ __global__ __launch_bounds__(kThreads) void k_findk(const SegDesc* __restrict__ 
   // multiples of 4 words at 16-byte aligned offsets.  The bucket prefixes
   // stay in registers until the group bases are scanned and are stored once,
   // base added.  Entries past the segment's bucket count are never used.
-  constexpr uint32_t TV = kMaxBuckets / 4 / kThreads;
   const uint32_t nt4 = sd.bmax / 4, ng4 = sd.bmax / kReduceBins / 4, nl4 = (1u << sd.pbits) / 4;
   const uint4* tot4 = reinterpret_cast<const uint4*>(totals + sd.tot_off);
   uint4 tv[TV];
@@ -915,6 +929,28 @@ __global__ __launch_bounds__(kThreads) void k_findk(const SegDesc* __restrict__ 
   }
   if (tid == 0) states[blockIdx.x].nact = U;
 }
+
+#define GGKS_FINDK_PARAMS                                                                      \
+  const SegDesc *__restrict__ segs, SegState *__restrict__ states,                             \
+      const uint32_t *__restrict__ lut_pool, const uint32_t *__restrict__ totals,              \
+      const uint32_t *__restrict__ gsums, const Query *__restrict__ queries,                   \
+      const uint32_t *__restrict__ kraw, uint32_t *__restrict__ status, uint32_t *__restrict__ qb, \
+      uint32_t *__restrict__ qr, Child *__restrict__ children, Query *__restrict__ cq,         \
+      uint32_t *__restrict__ bits_pool, uint32_t *__restrict__ wbase_pool, KT *__restrict__ out
+#define GGKS_FINDK_ARGS \
+  segs, states, lut_pool, totals, gsums, queries, kraw, status, qb, qr, children, cq, bits_pool, wbase_pool, out
+constexpr uint32_t kFindkSmallBuckets = 4 * kThreads;  // one 16-byte word per thread
+template <typename KT, int C>
+__global__ __launch_bounds__(kThreads) void k_findk(GGKS_FINDK_PARAMS) {
+  findk_segment<KT, C, kMaxBuckets / 4 / kThreads>(GGKS_FINDK_ARGS);
+}
+template <typename KT, int C>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) void k_findk_small(
+    GGKS_FINDK_PARAMS) {
+  findk_segment<KT, C, 1>(GGKS_FINDK_ARGS);
+}
+#undef GGKS_FINDK_PARAMS
+#undef GGKS_FINDK_ARGS
 
 // Pass 2 (k_compact): stream the chunk once and move the keys of active
 // buckets into their children's ranges (each chunk owns a private slice of
@@ -1160,6 +1196,13 @@ constexpr uint64_t kWaveMaxLen = 1ull << 15;
 __device__ __forceinline__ bool wave_child(const Child& c) {
   return c.q_cnt <= kWaveRanks && c.len <= kWaveMaxLen;
 }
+// Children with more than kSortRanks ranks and at most kSortCap keys (top-K
+// buckets, mostly) are sorted by k_finish_small too; k_finish takes what
+// fits neither.
+constexpr uint32_t kSortCap = 1024;
+__device__ __forceinline__ bool sort_child(const Child& c) {
+  return !wave_child(c) && c.q_cnt > kSortRanks && c.len <= kSortCap;
+}
 constexpr uint32_t kFThreads = 256;
 constexpr uint32_t kFDigit = 11;                      // first digit: 2048 bins
 constexpr uint32_t kFWords = (1u << kFDigit) + (1u << kFDigit) / 32;  // padded: bin b at b + b/32
@@ -1281,15 +1324,17 @@ __device__ __forceinline__ void wave_pick256(const uint32_t* h, uint32_t lane, u
   below = (uint32_t)__shfl((int)start, src, 64);
 }
 
-// Finisher for children with at most kWaveRanks ranks: one 256-thread
-// work-group per child (several per CU), two passes over the child's keys in
-// memory.  Pass 1 builds an 11-bit top-digit histogram of (key - lo) in a
-// padded LDS table (lane l's 32 bins are bank disjoint); wavefront 0 then
-// finds each rank's top digit.  Pass 2 gathers, per rank, the keys holding
-// that digit (a handful, as the child is one bucket) into LDS, where
-// wavefront 0 resolves the rank: by counting for up to 64 keys, by 8-bit
-// radix passes over LDS up to the gather capacity, and beyond it (heavy
-// duplicates of one digit) by 8-bit radix passes over the child in memory.
+// Finisher for children with at most kWaveRanks ranks, and for children
+// with many ranks that fit kSortCap keys: one 256-thread work-group per child
+// (several per CU).  The first kind takes two passes over the child's keys
+// (held in registers when they fit).  Pass 1 builds an 11-bit top-digit
+// histogram of (key - lo) in a padded LDS table (lane l's 32 bins are bank
+// disjoint); wavefront 0 then finds each rank's top digit.  Pass 2 gathers,
+// per rank, the keys holding that digit (a handful, as the child is one
+// bucket) into LDS, where wavefront 0 resolves the rank: by counting for up
+// to 64 keys, by 8-bit radix passes over LDS up to the gather capacity, and
+// beyond it (heavy duplicates of one digit) by 8-bit radix passes over the
+// child in memory.  The second kind is sorted in LDS and every rank read off.
 template <typename KT, int C>
 __global__ __launch_bounds__(kFThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_finish_small(const KT* __restrict__ input,
                                                             const KT* __restrict__ red,
@@ -1299,7 +1344,8 @@ __global__ __launch_bounds__(kFThreads) __attribute__((amdgpu_waves_per_eu(4))) 
                                                             KT* __restrict__ out) {
   constexpr uint32_t W = KeyBits<KT>::W;
   constexpr uint32_t CAP = kFCollectWords * 4 / sizeof(KT);  // gathered keys
-  __shared__ uint32_t h[kFWords];
+  static_assert(kSortCap * sizeof(KT) <= kFWords * 4, "a sorted child fits the histogram table");
+  __shared__ __align__(16) uint32_t h[kFWords];  // histogram, gathered keys, or a child to sort
   __shared__ KT red_lo[kFThreads / 64], red_hi[kFThreads / 64];
   __shared__ uint32_t res[4 * kWaveRanks + 2];  // per rank: digit, residual, count; gathered
   const uint32_t tid = threadIdx.x, lane = lane_id();
@@ -1318,13 +1364,44 @@ __global__ __launch_bounds__(kFThreads) __attribute__((amdgpu_waves_per_eu(4))) 
     if (wstart + tid < ncand) {
       const uint32_t ci = blockIdx.x + (wstart + tid) * gridDim.x;
       const Child cc = children[ci];
-      if (cc.q_cnt != 0 && wave_child(cc)) list[atomicAdd(&list[kFThreads], 1u)] = ci;
+      if (cc.q_cnt != 0 && (wave_child(cc) || sort_child(cc)))
+        list[atomicAdd(&list[kFThreads], 1u)] = ci;
     }
     __syncthreads();
     const uint32_t nl = list[kFThreads];
     for (uint32_t li = 0; li < nl; ++li) {
       const Child c = children[list[li]];
       const KT* srcp = (c.src ? red : input) + c.start;
+      if (sort_child(c)) {  // uniform
+        // many ranks: the converted keys sorted ascending in LDS (bitonic,
+        // one compare-exchange pair per thread step); rank r is the r-th key
+        KT* keys = reinterpret_cast<KT*>(h);
+        const uint32_t len = (uint32_t)c.len;
+        uint32_t np = 1;
+        while (np < len) np <<= 1;
+        for (uint32_t i = tid; i < np; i += kFThreads)
+          keys[i] = i < len ? to_key<C>(srcp[i]) : KeyBits<KT>::MAX;
+        __syncthreads();
+        for (uint32_t kk = 2; kk <= np; kk <<= 1) {
+          for (uint32_t j = kk >> 1; j > 0; j >>= 1) {
+            for (uint32_t pr = tid; pr < np / 2; pr += kFThreads) {
+              const uint32_t i = ((pr & ~(j - 1)) << 1) | (pr & (j - 1)), ix = i | j;
+              const KT a = keys[i], b = keys[ix];
+              if ((a > b) == ((i & kk) == 0)) {
+                keys[i] = b;
+                keys[ix] = a;
+              }
+            }
+            __syncthreads();
+          }
+        }
+        for (uint32_t i = tid; i < c.q_cnt; i += kFThreads) {
+          const Query q = cq[c.q_off + i];
+          out[q.out] = from_key<C>(keys[q.rank - 1]);
+        }
+        __syncthreads();
+        continue;
+      }
       // the passes over the child's keys, given a visitor of its keys
       auto body = [&](auto&& each) {
         KT lo = (KT)c.lo, hi = (KT)c.hi;
@@ -1542,7 +1619,7 @@ __global__ __launch_bounds__(kThreads) void k_finish(const KT* __restrict__ inpu
     if (w0 + tid < ncand) {
       const uint32_t ci = blockIdx.x + (w0 + tid) * gridDim.x;
       const Child c = children[ci];
-      if (c.q_cnt != 0 && !wave_child(c)) list[atomicAdd(&nlist, 1u)] = ci;
+      if (c.q_cnt != 0 && !wave_child(c) && !sort_child(c)) list[atomicAdd(&nlist, 1u)] = ci;
     }
     __syncthreads();
     const uint32_t nl = nlist;

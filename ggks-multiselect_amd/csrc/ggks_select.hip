@@ -168,6 +168,8 @@ static void level_attrs(const Plan&) {
     set_lds_limit(k_count<KT, C>, ((size_t)kMaxBuckets + (1u << kMaxPrefixBits)) * 4);
     set_lds_limit(k_findk<KT, C>, ((size_t)kMaxBuckets + (1u << kMaxPrefixBits) +
                                    3 * ((kMaxBuckets + 31) / 32) + 32) * 4);
+    set_lds_limit(k_findk_small<KT, C>, ((size_t)kMaxBuckets + (1u << kMaxPrefixBits) +
+                                   3 * ((kMaxBuckets + 31) / 32) + 32) * 4);
     set_lds_limit(k_compact<KT, C>, compact_lds_bytes<false>());
     set_lds_limit(k_compact_small<KT, C>, compact_lds_bytes<true>());
     set_lds_limit(k_finish<KT, C>, kFinishLdsTotal);
@@ -221,7 +223,7 @@ static int enqueue_level(const Exec& x, const Plan& p, const KT* d_src, KT* d_ou
     stage(x, 1);
     if (!p.sblocks.empty()) {
       const SampleBlock* sblocks = reinterpret_cast<const SampleBlock*>(A + p.off_sblocks);
-      hipLaunchKernelGGL((k_sample<KT, C>), dim3((uint32_t)p.sblocks.size()), dim3(kThreads), 0, s,
+      hipLaunchKernelGGL((k_sample<KT, C>), dim3((uint32_t)p.sblocks.size()), dim3(kSampleThreads), 0, s,
                          d_src, segs, sblocks, samples, minmax, p.seed);
     }
     stage(x, 2);
@@ -235,11 +237,18 @@ static int enqueue_level(const Exec& x, const Plan& p, const KT* d_src, KT* d_ou
     hipLaunchKernelGGL(k_reduce, dim3((uint32_t)p.items.size()), dim3(kThreads), 0, s, segs,
                        states, items, rows, tot, gsum);
     stage(x, 5);
-    hipLaunchKernelGGL((k_findk<KT, C>), dim3(nseg), dim3(kThreads),
-                       ((size_t)p.bmax_max + (1u << p.pbits_max) + 3 * ((p.bmax_max + 31) / 32) +
-                        32) * 4,
-                       s, segs, states, lut, tot, gsum, q, d_k, x.status, qb, qr, kids, cq, bits, wbase,
-                       d_out);
+    if (p.bmax_max <= kFindkSmallBuckets)
+      hipLaunchKernelGGL((k_findk_small<KT, C>), dim3(nseg), dim3(kThreads),
+                         ((size_t)p.bmax_max + (1u << p.pbits_max) + 3 * ((p.bmax_max + 31) / 32) +
+                          32) * 4,
+                         s, segs, states, lut, tot, gsum, q, d_k, x.status, qb, qr, kids, cq, bits, wbase,
+                         d_out);
+    else
+      hipLaunchKernelGGL((k_findk<KT, C>), dim3(nseg), dim3(kThreads),
+                         ((size_t)p.bmax_max + (1u << p.pbits_max) + 3 * ((p.bmax_max + 31) / 32) +
+                          32) * 4,
+                         s, segs, states, lut, tot, gsum, q, d_k, x.status, qb, qr, kids, cq, bits, wbase,
+                         d_out);
     stage(x, 6);
     if (compact_small(p.pbits_max, p.bmax_max))
       hipLaunchKernelGGL((k_compact_small<KT, C>), dim3(nchunks), dim3(kThreads),
