@@ -136,6 +136,40 @@ __device__ __forceinline__ void strided_for(const T* __restrict__ p, uint32_t n,
   for (; i < n; i += kThreads) f(i, p[i]);
 }
 
+// Ascending bitonic sort of keys[0, np) in LDS (np a power of two) by a
+// work-group of NT threads.  In every stage a thread owns up to MAXP of the
+// np / 2 disjoint compare-exchange pairs: it loads all of them, then
+// compares and stores, so its LDS round trips overlap.
+template <uint32_t NT, uint32_t MAXP, typename KT>
+__device__ __forceinline__ void lds_bitonic_sort(KT* keys, uint32_t np) {
+  for (uint32_t kk = 2; kk <= np; kk <<= 1) {
+    for (uint32_t j = kk >> 1; j > 0; j >>= 1) {
+      KT a[MAXP], b[MAXP];
+#pragma unroll
+      for (uint32_t u = 0; u < MAXP; ++u) {
+        const uint32_t pr = threadIdx.x + u * NT;
+        if (pr < np / 2) {
+          const uint32_t i = ((pr & ~(j - 1)) << 1) | (pr & (j - 1));
+          a[u] = keys[i];
+          b[u] = keys[i | j];
+        }
+      }
+#pragma unroll
+      for (uint32_t u = 0; u < MAXP; ++u) {
+        const uint32_t pr = threadIdx.x + u * NT;
+        if (pr < np / 2) {
+          const uint32_t i = ((pr & ~(j - 1)) << 1) | (pr & (j - 1));
+          if ((a[u] > b[u]) == ((i & kk) == 0)) {
+            keys[i] = b[u];
+            keys[i | j] = a[u];
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
 template <typename KT>
 __device__ __forceinline__ uint32_t mulhi_nb(KT x, uint32_t nb);
 template <>
@@ -1382,19 +1416,7 @@ __global__ __launch_bounds__(kFThreads) __attribute__((amdgpu_waves_per_eu(4))) 
         for (uint32_t i = tid; i < np; i += kFThreads)
           keys[i] = i < len ? to_key<C>(srcp[i]) : KeyBits<KT>::MAX;
         __syncthreads();
-        for (uint32_t kk = 2; kk <= np; kk <<= 1) {
-          for (uint32_t j = kk >> 1; j > 0; j >>= 1) {
-            for (uint32_t pr = tid; pr < np / 2; pr += kFThreads) {
-              const uint32_t i = ((pr & ~(j - 1)) << 1) | (pr & (j - 1)), ix = i | j;
-              const KT a = keys[i], b = keys[ix];
-              if ((a > b) == ((i & kk) == 0)) {
-                keys[i] = b;
-                keys[ix] = a;
-              }
-            }
-            __syncthreads();
-          }
-        }
+        lds_bitonic_sort<kFThreads, kSortCap / 2 / kFThreads>(keys, np);
         for (uint32_t i = tid; i < c.q_cnt; i += kFThreads) {
           const Query q = cq[c.q_off + i];
           out[q.out] = from_key<C>(keys[q.rank - 1]);
@@ -1663,19 +1685,7 @@ __global__ __launch_bounds__(kThreads) void k_finish(const KT* __restrict__ inpu
         while (np < c.len) np <<= 1;
         for (uint32_t i = c.len + tid; i < np; i += kThreads) keys[i] = KeyBits<KT>::MAX;
         __syncthreads();
-        for (uint32_t kk = 2; kk <= np; kk <<= 1) {
-          for (uint32_t j = kk >> 1; j > 0; j >>= 1) {
-            for (uint32_t i = tid; i < np; i += kThreads) {
-              uint32_t ix = i ^ j;
-              if (ix > i) {
-                KT a = keys[i], b = keys[ix];
-                bool up = (i & kk) == 0;
-                if ((a > b) == up) { keys[i] = b; keys[ix] = a; }
-              }
-            }
-            __syncthreads();
-          }
-        }
+        lds_bitonic_sort<kThreads, CAP / 2 / kThreads>(keys, np);
         for (uint32_t i = tid; i < c.q_cnt; i += kThreads) {
           Query q = cq[c.q_off + i];
           out[q.out] = from_key<C>((KT)(lo + keys[q.rank - 1]));
@@ -1698,19 +1708,7 @@ __global__ __launch_bounds__(kThreads) void k_finish(const KT* __restrict__ inpu
           }
         }
         __syncthreads();
-        for (uint32_t kk = 2; kk <= np; kk <<= 1) {
-          for (uint32_t j = kk >> 1; j > 0; j >>= 1) {
-            for (uint32_t i = tid; i < np; i += kThreads) {
-              uint32_t ix = i ^ j;
-              if (ix > i) {
-                uint64_t a = qv[i], b = qv[ix];
-                bool up = (i & kk) == 0;
-                if ((a > b) == up) { qv[i] = b; qv[ix] = a; }
-              }
-            }
-            __syncthreads();
-          }
-        }
+        lds_bitonic_sort<kThreads, (kQCap / 2 + kThreads - 1) / kThreads>(qv, np);
         for (uint32_t i = tid; i < nq; i += kThreads) qpre[i] = 0;
         __syncthreads();
 
