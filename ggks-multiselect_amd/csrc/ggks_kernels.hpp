@@ -14,8 +14,14 @@
 //             child per active bucket and group the ranks by child
 //   k_compact pass 2 over HBM: copy only keys of active buckets, each chunk
 //             into its private slice of the child's range
-//   k_finish  per child: multi-rank MSD radix select on 8-bit digits of
-//             (key - lo), keys resident in LDS when they fit
+//   k_finish_small  per child with few ranks: top-digit histogram, then the
+//             rank's digit gathered and ranked (keys in registers when they
+//             fit); per child of up to 1024 keys with many ranks: LDS sort
+//   k_finish  every other child: multi-rank MSD radix select on 8-bit digits
+//             of (key - lo), or a sort, keys resident in LDS when they fit
+// Plans of at most 4096 buckets per segment run k_findk_small and
+// k_compact_small: the same bodies with smaller tables, two work-groups per
+// CU.
 //
 // Reference counterparts: bucketMultiselect.cu phaseOne (:521-833) --
 // assignSmartBucket (:129-205) -> k_count, sumCounts (:214-221) -> k_reduce,

@@ -168,8 +168,8 @@ static void level_attrs(const Plan&) {
     set_lds_limit(k_count<KT, C>, ((size_t)kMaxBuckets + (1u << kMaxPrefixBits)) * 4);
     set_lds_limit(k_findk<KT, C>, ((size_t)kMaxBuckets + (1u << kMaxPrefixBits) +
                                    3 * ((kMaxBuckets + 31) / 32) + 32) * 4);
-    set_lds_limit(k_findk_small<KT, C>, ((size_t)kMaxBuckets + (1u << kMaxPrefixBits) +
-                                   3 * ((kMaxBuckets + 31) / 32) + 32) * 4);
+    set_lds_limit(k_findk_small<KT, C>, ((size_t)kFindkSmallBuckets + (1u << kMaxPrefixBits) +
+                                         3 * ((kFindkSmallBuckets + 31) / 32) + 32) * 4);
     set_lds_limit(k_compact<KT, C>, compact_lds_bytes<false>());
     set_lds_limit(k_compact_small<KT, C>, compact_lds_bytes<true>());
     set_lds_limit(k_finish<KT, C>, kFinishLdsTotal);
@@ -179,9 +179,11 @@ static void level_attrs(const Plan&) {
 }
 
 // Both finishers over a child list: a 256-thread work-group per child with
-// few ranks, a 1024-thread work-group per child otherwise (each kernel skips the other's).  The
-// work-group finisher's LDS allows one group per CU, so a grid of max_groups
-// (256: one per CU) loops over the list without losing concurrency.
+// few ranks (or many ranks and at most 1024 keys), a 1024-thread work-group
+// per child otherwise; each kernel screens out the other's children.  The
+// 1024-thread finisher's LDS allows one group per CU, so a grid of
+// max_groups (256: one per CU) loops over the list without losing
+// concurrency.
 template <typename KT, int C>
 static void finish(hipStream_t s, const KT* d_src, const KT* red, const Child* kids, uint32_t n,
                    const Query* cq, KT* d_out, uint32_t max_groups) {
