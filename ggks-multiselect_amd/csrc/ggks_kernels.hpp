@@ -1236,13 +1236,16 @@ constexpr uint64_t kWaveMaxLen = 1ull << 15;
 __device__ __forceinline__ bool wave_child(const Child& c) {
   return c.q_cnt <= kWaveRanks && c.len <= kWaveMaxLen;
 }
-// Children with more than kSortRanks ranks and at most kSortCap keys (top-K
+// Children with more than kWaveRanks ranks and at most kSortCap keys (top-K
 // buckets, mostly) are sorted by k_finish_small too; k_finish takes what
 // fits neither.
 constexpr uint32_t kSortCap = 1024;
 __device__ __forceinline__ bool sort_child(const Child& c) {
-  return !wave_child(c) && c.q_cnt > kSortRanks && c.len <= kSortCap;
+  return !wave_child(c) && c.len <= kSortCap;
 }
+// k_finish sorts a child held in LDS when it has many ranks or is short
+// enough that a sort beats the digit passes.
+constexpr uint32_t kSortLen = 4096;
 constexpr uint32_t kFThreads = 256;
 constexpr uint32_t kFDigit = 11;                      // first digit: 2048 bins
 constexpr uint32_t kFWords = (1u << kFDigit) + (1u << kFDigit) / 32;  // padded: bin b at b + b/32
@@ -1685,8 +1688,8 @@ __global__ __launch_bounds__(kThreads) void k_finish(const KT* __restrict__ inpu
       }
       const int nbits = bitlen<KT>(hi - lo);
 
-      if (inl && c.q_cnt > kSortRanks) {
-        // many ranks: sort the child's keys in LDS, then every rank is a lookup
+      if (inl && (c.q_cnt > kSortRanks || c.len <= kSortLen)) {
+        // many ranks or a short child: sort its keys in LDS, then every rank is a lookup
         uint32_t np = 1;
         while (np < c.len) np <<= 1;
         for (uint32_t i = c.len + tid; i < np; i += kThreads) keys[i] = KeyBits<KT>::MAX;
