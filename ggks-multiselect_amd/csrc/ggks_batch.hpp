@@ -16,6 +16,8 @@ struct ggks_batch {
   int dtype = 0;
   hipStream_t stream = nullptr;
   bool own_stream = false;
+  hipStream_t side = nullptr;  // direct children finish here, beside the engine pipeline
+  hipEvent_t fork = nullptr, join = nullptr;
   ggks::Plan plan;
   char* arena = nullptr;  // plan tables + scratch + status word
   bool own_arena = false;
@@ -95,7 +97,7 @@ static int batch_enqueue_typed(const Exec& x, const Plan& p, const void* src, vo
 }
 
 static int batch_enqueue(ggks_batch* b, const void* src, void* out) {
-  const Exec x = {b->arena, b->stream, b->status, nullptr};
+  const Exec x = {b->arena, b->stream, b->status, nullptr, b->side, b->fork, b->join};
   const uint32_t* d_k = reinterpret_cast<const uint32_t*>(b->arena + b->plan.off_kvals);
   switch (b->dtype) {
     case GGKS_F32: return batch_enqueue_typed<uint32_t, CONV_F>(x, b->plan, src, out, d_k);
@@ -140,6 +142,9 @@ static int batch_create(ggks_batch** out, int device, int dtype, const ggks_segm
   auto bail = [&](int code) {
     if (b->own_arena && b->arena) (void)hipFree(b->arena);
     if (b->own_stream && b->stream) (void)hipStreamDestroy(b->stream);
+    if (b->side) (void)hipStreamDestroy(b->side);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    if (b->join) (void)hipEventDestroy(b->join);
     delete b;
     return code;
   };
@@ -167,6 +172,14 @@ static int batch_create(ggks_batch** out, int device, int dtype, const ggks_segm
       return bail(fail(GGKS_EDEVICE, "cannot create a stream"));
     }
     b->own_stream = true;
+  }
+  if (b->engine > 0 && b->direct > 0) {  // default priority, like every stream here
+    if (hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->join, hipEventDisableTiming) != hipSuccess) {
+      (void)hipGetLastError();
+      return bail(fail(GGKS_EDEVICE, "cannot create the batch's side stream"));
+    }
   }
   const size_t tb = b->plan.table_bytes();
   std::vector<char> tab(tb + 256, 0);
@@ -236,6 +249,9 @@ static int batch_destroy(ggks_batch* b) {
   batch_drop_graph(b);
   if (b->own_arena && b->arena) (void)hipFree(b->arena);
   if (b->own_stream && b->stream) (void)hipStreamDestroy(b->stream);
+  if (b->side) (void)hipStreamDestroy(b->side);
+  if (b->fork) (void)hipEventDestroy(b->fork);
+  if (b->join) (void)hipEventDestroy(b->join);
   delete b;
   return GGKS_OK;
 }

@@ -1659,8 +1659,21 @@ __global__ __launch_bounds__(kThreads) void k_finish(const KT* __restrict__ inpu
       const KT* srcp = (c.src ? red : input) + c.start;
       const bool inl = c.len <= CAP;
       KT lo = (KT)c.lo, hi = (KT)c.hi;
-      if (inl) {
-        for (uint32_t i = tid; i < c.len; i += kThreads) keys[i] = to_key<C>(srcp[i]);
+      if (inl) {  // the child into LDS, eight loads per thread in flight
+        constexpr uint32_t D = 8;
+        for (uint32_t i0 = 0; i0 < c.len; i0 += D * kThreads) {
+          KT v[D];
+#pragma unroll
+          for (uint32_t u = 0; u < D; ++u) {
+            const uint32_t i = i0 + u * kThreads + tid;
+            if (i < c.len) v[u] = srcp[i];
+          }
+#pragma unroll
+          for (uint32_t u = 0; u < D; ++u) {
+            const uint32_t i = i0 + u * kThreads + tid;
+            if (i < c.len) keys[i] = to_key<C>(v[u]);
+          }
+        }
       }
       if (c.need_range) {
         KT a = KeyBits<KT>::MAX, b = 0;

@@ -152,6 +152,10 @@ struct Exec {
   hipStream_t stream;
   uint32_t* status;
   ggks_ctx* prof;  // null: no stage events (also during graph capture)
+  // optional: a second stream (and fork/join events) on which the direct
+  // children are finished beside the engine pipeline
+  hipStream_t side = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
 };
 
 static void stage(const Exec& x, int st) {
@@ -193,13 +197,30 @@ static void finish(hipStream_t s, const KT* d_src, const KT* red, const Child* k
                      kFinishLdsTotal, s, d_src, red, kids, n, cq, d_out);
 }
 
-// The level pipeline over every engine segment of the plan, then the
-// finisher over the plan's direct children (whole small segments).
+// The level pipeline over every engine segment of the plan, and the
+// finisher over the plan's direct children (whole small segments): after
+// the pipeline, or, with a side stream, beside it from the start (the two
+// read the input and write disjoint outputs; the side stream is joined back
+// before returning, also under graph capture, where it forms a parallel
+// branch).
 template <typename KT, int C>
 static int enqueue_level(const Exec& x, const Plan& p, const KT* d_src, KT* d_out,
                          const uint32_t* d_k) {
   char* A = x.arena;
   hipStream_t s = x.stream;
+  const uint32_t ndirect = (uint32_t)p.direct_children.size();
+  const bool beside = ndirect > 0 && !p.segs.empty() && x.side;
+  auto finish_direct = [&](hipStream_t ds) {
+    const Child* dk = reinterpret_cast<const Child*>(A + p.off_dchildren);
+    const Query* dq = reinterpret_cast<const Query*>(A + p.off_dqueries);
+    finish<KT, C>(ds, d_src, (const KT*)nullptr, dk, ndirect, dq, d_out, 256);
+  };
+  if (beside) {
+    HIP_TRY(hipEventRecord(x.fork, s));
+    HIP_TRY(hipStreamWaitEvent(x.side, x.fork, 0));
+    finish_direct(x.side);
+    HIP_TRY(hipEventRecord(x.join, x.side));
+  }
   const uint32_t nseg = (uint32_t)p.segs.size();
   if (nseg > 0) {
     const SegDesc* segs = reinterpret_cast<const SegDesc*>(A + p.off_segs);
@@ -265,11 +286,10 @@ static int enqueue_level(const Exec& x, const Plan& p, const KT* d_src, KT* d_ou
   } else {
     for (int st = 1; st < GGKS_NSTAGES; ++st) stage(x, st);
   }
-  const uint32_t ndirect = (uint32_t)p.direct_children.size();
-  if (ndirect > 0) {
-    const Child* dk = reinterpret_cast<const Child*>(A + p.off_dchildren);
-    const Query* dq = reinterpret_cast<const Query*>(A + p.off_dqueries);
-    finish<KT, C>(s, d_src, (const KT*)nullptr, dk, ndirect, dq, d_out, 256);
+  if (beside) {
+    HIP_TRY(hipStreamWaitEvent(s, x.join, 0));
+  } else if (ndirect > 0) {
+    finish_direct(s);
   }
   HIP_TRY(hipGetLastError());
   return GGKS_OK;
