@@ -28,7 +28,7 @@ EXPORTS = [
     "ggks_ctx_reserve", "ggks_ctx_check", "ggks_ctx_stream", "ggks_dev_alloc",
     "ggks_dev_free", "ggks_h2d", "ggks_d2h", "ggks_d2d", "ggks_sync",
     "ggks_device_count", "ggks_set_device", "ggks_dev_fill", "ggks_ctx_profile_arm",
-    "ggks_ctx_profile_read", "ggks_batch_create", "ggks_batch_scratch_bytes", "ggks_batch_run",
+    "ggks_ctx_profile_read", "ggks_ctx_profile_marks", "ggks_batch_create", "ggks_batch_scratch_bytes", "ggks_batch_run",
     "ggks_batch_sync", "ggks_batch_stream", "ggks_batch_info", "ggks_batch_destroy",
     "ggks_tuning_set", "ggks_tuning_get", "ggks_tuning_reset", "ggks_generate",
     "ggks_server_create", "ggks_server_submit", "ggks_server_poll", "ggks_server_drain",
@@ -137,6 +137,7 @@ def lib() -> ctypes.CDLL:
         "ggks_dev_fill": ([vp, i32, sz], i32),
         "ggks_ctx_profile_arm": ([vp, i32], i32),
         "ggks_ctx_profile_read": ([vp, vp, i32], i32),
+        "ggks_ctx_profile_marks": ([vp, ctypes.c_uint32], i32),
         "ggks_batch_create": ([ctypes.POINTER(vp), i32, i32, vp, i64, vp, i64, optp, vp, sz, vp],
                               i32),
         "ggks_batch_scratch_bytes": ([i32, vp, i64, vp, i64, optp], sz),

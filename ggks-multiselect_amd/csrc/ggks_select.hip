@@ -62,6 +62,7 @@ struct ggks_ctx {
   int prof_slots = 0;
   uint64_t prof_calls = 0;
   int prof_cur = -1;  // slot of the call in flight, -1 when not armed
+  uint32_t prof_mask = ~0u;  // events recorded (ggks_ctx_profile_marks)
   // plan-table cache: the tables in the arena match this signature
   uint64_t table_sig = 0;
   bool pinned_busy = false;  // an async copy from `pinned` may be in flight
@@ -133,7 +134,7 @@ static int default_ctx(ggks_ctx** out) {
 // Record the event that opens `stage` (GGKS_NSTAGES closes the call).  Stages
 // a call skips are recorded at the same point, so they measure zero.
 static void mark(ggks_ctx* c, int stage) {
-  if (c->prof_cur < 0) return;
+  if (c->prof_cur < 0 || !((c->prof_mask >> stage) & 1u)) return;
   const int per = GGKS_NSTAGES + 1;
   (void)hipEventRecord(c->prof[(size_t)c->prof_cur * per + stage], c->stream);
 }
@@ -494,12 +495,34 @@ int ggks_ctx_profile_read(ggks_ctx* c, float* ms, int nslots) {
   HIP_TRY(hipSetDevice(c->device));
   HIP_TRY(hipStreamSynchronize(c->stream));
   const int per = GGKS_NSTAGES + 1;
+  const uint32_t mask = c->prof_mask;
+  auto recorded = [&](int m) { return ((mask >> m) & 1u) != 0; };
+  int first = -1, last = -1;
+  for (int m = 0; m <= GGKS_NSTAGES; ++m)
+    if (recorded(m)) {
+      if (first < 0) first = m;
+      last = m;
+    }
   for (int sl = 0; sl < nslots; ++sl) {
     hipEvent_t* e = &c->prof[(size_t)sl * per];
-    for (int st = 0; st < GGKS_NSTAGES; ++st)
-      HIP_TRY(hipEventElapsedTime(&ms[sl * per + st], e[st], e[st + 1]));
-    HIP_TRY(hipEventElapsedTime(&ms[sl * per + GGKS_NSTAGES], e[0], e[GGKS_NSTAGES]));
+    float* row = ms + (size_t)sl * per;
+    for (int st = 0; st <= GGKS_NSTAGES; ++st) row[st] = 0.f;
+    for (int st = 0; st < GGKS_NSTAGES; ++st) {
+      if (!recorded(st)) continue;
+      int nx = st + 1;
+      while (nx <= GGKS_NSTAGES && !recorded(nx)) ++nx;
+      if (nx <= GGKS_NSTAGES) HIP_TRY(hipEventElapsedTime(&row[st], e[st], e[nx]));
+    }
+    if (first >= 0 && last > first)
+      HIP_TRY(hipEventElapsedTime(&row[GGKS_NSTAGES], e[first], e[last]));
   }
+  return GGKS_OK;
+}
+
+int ggks_ctx_profile_marks(ggks_ctx* c, uint32_t mask) {
+  if (!c) return fail(GGKS_EINVAL, "null ctx");
+  std::lock_guard<std::mutex> lk(c->mu);
+  c->prof_mask = mask & ((1u << (GGKS_NSTAGES + 1)) - 1u);
   return GGKS_OK;
 }
 

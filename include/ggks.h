@@ -143,6 +143,14 @@ void *ggks_ctx_stream(ggks_ctx *ctx);
 #define GGKS_NSTAGES 8
 int ggks_ctx_profile_arm(ggks_ctx *ctx, int nslots);
 int ggks_ctx_profile_read(ggks_ctx *ctx, float *ms, int nslots);
+/* Record only the events in `mask` (bit m: the event before stage m; bit
+ * GGKS_NSTAGES: the call's last event; default all).  Every event costs
+ * device time between the kernels it separates, so a timed run records only
+ * the two around the kernel it times.  ggks_ctx_profile_read() then reports
+ * for a recorded event m the time to the next recorded one in ms[.. + m],
+ * 0 for the others, and the first to the last recorded event as the whole
+ * call.  Takes effect from the next select. */
+int ggks_ctx_profile_marks(ggks_ctx *ctx, uint32_t mask);
 
 /* ---- device plumbing for hosts without a GPU framework --------------
  * Thin wrappers over the HIP runtime this library is linked with. */

@@ -162,6 +162,43 @@ def test_async_entry_and_bad_k_status(pkg, O):
     assert L.lib().ggks_ctx_check(ctx.handle) == L.OK  # cleared
 
 
+def test_stage_events_and_marks(pkg, O):
+    """Stage timing: all events give every stage and a whole-call time equal
+    to their sum; a mask of the two events around the compaction pass gives
+    that stage alone (the others read 0) and the same outputs."""
+    from ggks_multiselect_amd import _lib as L
+    lib = L.lib()
+    W = pkg.workloads
+    n = 1 << 22
+    x = W.normal_f32(6, n)
+    k = klist(W, n, 6, count=64)
+    d = pkg.DeviceArray.from_host(x)
+    dk = pkg.DeviceArray.from_host(k)
+    dout = pkg.DeviceArray(k.size, np.float32)
+    ctx = pkg.Context(0)
+    per = L.NSTAGES + 1
+    ci = L.STAGES.index("compact")
+    for mask in ((1 << per) - 1, (1 << ci) | (1 << (ci + 1))):
+        L.check(lib.ggks_ctx_profile_marks(ctx.handle, mask))
+        L.check(lib.ggks_ctx_profile_arm(ctx.handle, 2))
+        for _ in range(2):
+            pkg.multiselect_async(d, dk, dout, ctx)
+        pkg.sync()
+        ms = np.zeros(2 * per, np.float32)
+        L.check(lib.ggks_ctx_profile_read(ctx.handle, ms.ctypes.data, 2))
+        ms = ms.reshape(2, per)
+        assert np.array_equal(bits(dout.to_host()), bits(O.sort_choose(x, k)))
+        for row in ms:
+            if mask == (1 << per) - 1:
+                assert row[ci] > 0 and row[L.STAGES.index("count")] > 0
+                assert abs(row[:L.NSTAGES].sum() - row[L.NSTAGES]) <= 1e-3 * row[L.NSTAGES] + 1e-3
+            else:
+                assert row[ci] > 0 and row[L.NSTAGES] == row[ci]
+                assert not np.any(np.delete(row[:L.NSTAGES], ci))
+    L.check(lib.ggks_ctx_profile_arm(ctx.handle, 0))
+    L.check(lib.ggks_ctx_profile_marks(ctx.handle, (1 << per) - 1))
+
+
 def test_options_do_not_change_results(pkg, O):
     W = pkg.workloads
     n = (1 << 22) + 1
