@@ -8,8 +8,8 @@ stream-ordered C-ABI entry point (device k list, device outputs); the timed
 region is K back-to-back steps between two stream synchronisations.  Inside
 it the library records HIP events on its stream only around the dominant
 kernel; a short untimed pass before it records an event between every two
-kernels for the stage breakdown (each event costs device time, about 30 us
-per step for all nine).
+kernels for the stage breakdown (each event costs device time: about 14 us
+per step for all nine, 2 us for the two of the timed region).
 
 Prints ONE JSON line.  value = n / wall time per step (Gelem/s).  roofline is
 for the dominant kernel (the larger of the two streaming passes, each of which

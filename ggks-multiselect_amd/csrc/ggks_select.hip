@@ -478,10 +478,14 @@ int ggks_ctx_profile_arm(ggks_ctx* c, int nslots) {
   c->prof.clear();
   c->prof_slots = 0;
   c->prof_calls = 0;
+  // timing-only events: no system-scope fence (cache writeback and
+  // invalidate) when they complete, which would cost device time between the
+  // kernels they separate; ggks_ctx_profile_read synchronises the stream
+  // before reading them
   const size_t n = (size_t)nslots * (GGKS_NSTAGES + 1);
   for (size_t i = 0; i < n; ++i) {
     hipEvent_t e;
-    HIP_TRY(hipEventCreate(&e));
+    HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
     c->prof.push_back(e);
   }
   c->prof_slots = nslots;
