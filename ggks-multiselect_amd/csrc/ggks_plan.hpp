@@ -305,8 +305,12 @@ inline void plan_batch(Plan& p, int dtype, const std::vector<BatchSeg>& in, cons
     csz = (csz + 63) / 64 * 64;
     G = (n + csz - 1) / csz;
     uint32_t ns = 0;
-    if (mode == MODE_SAMPLE) {  // one sample per 256 keys, 1024 to 65536
-      uint64_t want = (n / 256 + 1023) / 1024 * 1024;
+    // one sample per 1024 keys, 1024 to 65536: every sample is a random
+    // gather that fetches a whole line, and a batch of many segments pays
+    // for them in HBM bandwidth; sparser samples leave buckets a little less
+    // even, which costs less than the gathers save
+    if (mode == MODE_SAMPLE) {
+      uint64_t want = (n / 1024 + 1023) / 1024 * 1024;
       ns = (uint32_t)std::min<uint64_t>(65536, std::max<uint64_t>(1024, want));
       for (uint32_t y = 0; y < ns / kThreads; ++y) p.sblocks.push_back({(uint32_t)p.segs.size(), y});
     }
