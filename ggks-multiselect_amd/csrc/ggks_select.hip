@@ -200,10 +200,11 @@ static void finish(hipStream_t s, const KT* d_src, const KT* red, const Child* k
 
 // The level pipeline over every engine segment of the plan, and the
 // finisher over the plan's direct children (whole small segments): after
-// the pipeline, or, with a side stream, beside it from the start (the two
-// read the input and write disjoint outputs; the side stream is joined back
-// before returning, also under graph capture, where it forms a parallel
-// branch).
+// the pipeline, or, with a side stream, beside it from the streaming count
+// pass on (the two read the input and write disjoint outputs; the side
+// stream is joined back before returning, also under graph capture, where
+// it forms a parallel branch).  Forked any earlier, the direct finishers
+// hold CUs that the many one-work-group-per-segment LUT builds wait for.
 template <typename KT, int C>
 static int enqueue_level(const Exec& x, const Plan& p, const KT* d_src, KT* d_out,
                          const uint32_t* d_k) {
@@ -216,12 +217,6 @@ static int enqueue_level(const Exec& x, const Plan& p, const KT* d_src, KT* d_ou
     const Query* dq = reinterpret_cast<const Query*>(A + p.off_dqueries);
     finish<KT, C>(ds, d_src, (const KT*)nullptr, dk, ndirect, dq, d_out, 256);
   };
-  if (beside) {
-    HIP_TRY(hipEventRecord(x.fork, s));
-    HIP_TRY(hipStreamWaitEvent(x.side, x.fork, 0));
-    finish_direct(x.side);
-    HIP_TRY(hipEventRecord(x.join, x.side));
-  }
   const uint32_t nseg = (uint32_t)p.segs.size();
   if (nseg > 0) {
     const SegDesc* segs = reinterpret_cast<const SegDesc*>(A + p.off_segs);
@@ -254,6 +249,12 @@ static int enqueue_level(const Exec& x, const Plan& p, const KT* d_src, KT* d_ou
     hipLaunchKernelGGL((k_lut<KT>), dim3(nseg), dim3(kThreads), 0, s, segs, samples, minmax, states,
                        lut);
     stage(x, 3);
+    if (beside) {  // nseg > 0
+      HIP_TRY(hipEventRecord(x.fork, s));
+      HIP_TRY(hipStreamWaitEvent(x.side, x.fork, 0));
+      finish_direct(x.side);
+      HIP_TRY(hipEventRecord(x.join, x.side));
+    }
     hipLaunchKernelGGL((k_count<KT, C>), dim3(nchunks), dim3(kThreads),
                        ((size_t)p.bmax_max + (1u << p.pbits_max)) * 4, s, d_src, segs, states,
                        chunks, lut, rows);
