@@ -21,7 +21,8 @@
 //             of (key - lo), or a sort, keys resident in LDS when they fit
 // Plans of at most 4096 buckets per segment run k_findk_small and
 // k_compact_small: the same bodies with smaller tables, two work-groups per
-// CU.
+// CU.  Plans whose children are small run k_finish_small_lite (4-byte keys):
+// the same body with fewer keys in registers, six work-groups per CU.
 //
 // Reference counterparts: bucketMultiselect.cu phaseOne (:521-833) --
 // assignSmartBucket (:129-205) -> k_count, sumCounts (:214-221) -> k_reduce,
@@ -1341,6 +1342,15 @@ struct GroupKeys {
 // spilling for every key type)
 template <typename KT>
 constexpr int kFRegWords = sizeof(KT) == 4 ? 9 : 8;
+// ... and by k_finish_small_lite (4-byte keys): 4 words (16 KB of keys) in at
+// most 80 VGPRs, so that six work-groups share a CU instead of four (plans
+// whose children are small: batches of many segments, mostly).  The 8-byte
+// instance would spill at that size and is not built.
+constexpr int kFLiteWords = 4;
+template <typename KT>
+constexpr uint64_t finish_small_reg_keys(bool lite) {
+  return (uint64_t)(lite ? kFLiteWords : kFRegWords<KT>) * kFThreads * (16 / sizeof(KT));
+}
 
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1378,13 +1388,14 @@ __device__ __forceinline__ void wave_pick256(const uint32_t* h, uint32_t lane, u
 // to 64 keys, by 8-bit radix passes over LDS up to the gather capacity, and
 // beyond it (heavy duplicates of one digit) by 8-bit radix passes over the
 // child in memory.  The second kind is sorted in LDS and every rank read off.
-template <typename KT, int C>
-__global__ __launch_bounds__(kFThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_finish_small(const KT* __restrict__ input,
-                                                            const KT* __restrict__ red,
-                                                            const Child* __restrict__ children,
-                                                            uint32_t nchildren,
-                                                            const Query* __restrict__ cq,
-                                                            KT* __restrict__ out) {
+// R is the number of 16-byte words of a child a thread holds in registers.
+template <typename KT, int C, int R>
+__device__ __forceinline__ void finish_small_body(const KT* __restrict__ input,
+                                                  const KT* __restrict__ red,
+                                                  const Child* __restrict__ children,
+                                                  uint32_t nchildren,
+                                                  const Query* __restrict__ cq,
+                                                  KT* __restrict__ out) {
   constexpr uint32_t W = KeyBits<KT>::W;
   constexpr uint32_t CAP = kFCollectWords * 4 / sizeof(KT);  // gathered keys
   static_assert(kSortCap * sizeof(KT) <= kFWords * 4, "a sorted child fits the histogram table");
@@ -1589,8 +1600,8 @@ __global__ __launch_bounds__(kFThreads) __attribute__((amdgpu_waves_per_eu(4))) 
       };
       // from registers when the child fits (read from memory once), else from
       // memory in every pass
-      if (GroupKeys<KT, kFRegWords<KT>>::fits(srcp, c.len)) {  // uniform
-        GroupKeys<KT, kFRegWords<KT>> gk;
+      if (GroupKeys<KT, R>::fits(srcp, c.len)) {  // uniform
+        GroupKeys<KT, R> gk;
         gk.load(srcp, c.len);
         body([&](auto&& f) { gk.for_each(f); });
       } else {
@@ -1601,6 +1612,21 @@ __global__ __launch_bounds__(kFThreads) __attribute__((amdgpu_waves_per_eu(4))) 
     __syncthreads();  // every thread is done with the list before it is refilled
   }
 }
+
+#define GGKS_FINISH_PARAMS                                                                    \
+  const KT *__restrict__ input, const KT *__restrict__ red, const Child *__restrict__ children, \
+      uint32_t nchildren, const Query *__restrict__ cq, KT *__restrict__ out
+template <typename KT, int C>
+__global__ __launch_bounds__(kFThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_finish_small(
+    GGKS_FINISH_PARAMS) {
+  finish_small_body<KT, C, kFRegWords<KT>>(input, red, children, nchildren, cq, out);
+}
+template <typename KT, int C>
+__global__ __launch_bounds__(kFThreads) __attribute__((amdgpu_waves_per_eu(6))) void k_finish_small_lite(
+    GGKS_FINISH_PARAMS) {
+  finish_small_body<KT, C, kFLiteWords>(input, red, children, nchildren, cq, out);
+}
+#undef GGKS_FINISH_PARAMS
 
 // Finisher: exact multi-rank radix select inside each child.
 // LDS layout (byte offsets, all 16-aligned): keys | hist | queries | query

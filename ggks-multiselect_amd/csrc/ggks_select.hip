@@ -191,9 +191,18 @@ static void level_attrs(const Plan&) {
 // concurrency.
 template <typename KT, int C>
 static void finish(hipStream_t s, const KT* d_src, const KT* red, const Child* kids, uint32_t n,
-                   const Query* cq, KT* d_out, uint32_t max_groups) {
-  hipLaunchKernelGGL((k_finish_small<KT, C>), dim3(std::min<uint32_t>(n, 65536)), dim3(kFThreads),
-                     0, s, d_src, red, kids, n, cq, d_out);
+                   const Query* cq, KT* d_out, uint32_t max_groups, bool lite) {
+  bool launched = false;
+  if constexpr (sizeof(KT) == 4) {
+    if (lite) {
+      hipLaunchKernelGGL((k_finish_small_lite<KT, C>), dim3(std::min<uint32_t>(n, 65536)),
+                         dim3(kFThreads), 0, s, d_src, red, kids, n, cq, d_out);
+      launched = true;
+    }
+  }
+  if (!launched)
+    hipLaunchKernelGGL((k_finish_small<KT, C>), dim3(std::min<uint32_t>(n, 65536)),
+                       dim3(kFThreads), 0, s, d_src, red, kids, n, cq, d_out);
   hipLaunchKernelGGL((k_finish<KT, C>), dim3(std::min<uint32_t>(n, max_groups)), dim3(kThreads),
                      kFinishLdsTotal, s, d_src, red, kids, n, cq, d_out);
 }
@@ -212,10 +221,20 @@ static int enqueue_level(const Exec& x, const Plan& p, const KT* d_src, KT* d_ou
   hipStream_t s = x.stream;
   const uint32_t ndirect = (uint32_t)p.direct_children.size();
   const bool beside = ndirect > 0 && !p.segs.empty() && x.side;
+  // the register-light 256-thread finisher when every child a plan can
+  // make is expected to fit its registers twice over (a longer child is
+  // still finished exactly, from memory): direct children by their length,
+  // an engine segment's by its mean bucket size
+  const uint64_t lite_keys = finish_small_reg_keys<KT>(true);
+  uint64_t dmax = 0, emax = 0;
+  for (const Child& c : p.direct_children) dmax = std::max<uint64_t>(dmax, c.len);
+  for (const SegDesc& sd : p.segs)
+    emax = std::max<uint64_t>(emax, sd.mode == MODE_RADIX ? sd.len : (sd.len + sd.bmax - 1) / sd.bmax);
+  const bool dlite = dmax <= lite_keys, elite = 2 * emax <= lite_keys;
   auto finish_direct = [&](hipStream_t ds) {
     const Child* dk = reinterpret_cast<const Child*>(A + p.off_dchildren);
     const Query* dq = reinterpret_cast<const Query*>(A + p.off_dqueries);
-    finish<KT, C>(ds, d_src, (const KT*)nullptr, dk, ndirect, dq, d_out, 256);
+    finish<KT, C>(ds, d_src, (const KT*)nullptr, dk, ndirect, dq, d_out, 256, dlite);
   };
   const uint32_t nseg = (uint32_t)p.segs.size();
   if (nseg > 0) {
@@ -284,7 +303,7 @@ static int enqueue_level(const Exec& x, const Plan& p, const KT* d_src, KT* d_ou
                          compact_lds_bytes<false>(), s, d_src, segs, states, chunks, lut, rows,
                          bits, wbase, kids, red);
     stage(x, 7);
-    finish<KT, C>(s, d_src, red, kids, p.nq, cq, d_out, 256);
+    finish<KT, C>(s, d_src, red, kids, p.nq, cq, d_out, 256, elite);
   } else {
     for (int st = 1; st < GGKS_NSTAGES; ++st) stage(x, st);
   }
