@@ -62,7 +62,8 @@ struct ggks_ctx {
   int prof_slots = 0;
   uint64_t prof_calls = 0;
   int prof_cur = -1;  // slot of the call in flight, -1 when not armed
-  uint32_t prof_mask = ~0u;  // events recorded (ggks_ctx_profile_marks)
+  uint32_t prof_mask = ~0u;   // events to record (ggks_ctx_profile_marks)
+  uint32_t prof_armed = ~0u;  // the mask latched by the last arm: recorded and read
   // plan-table cache: the tables in the arena match this signature
   uint64_t table_sig = 0;
   bool pinned_busy = false;  // an async copy from `pinned` may be in flight
@@ -134,7 +135,7 @@ static int default_ctx(ggks_ctx** out) {
 // Record the event that opens `stage` (GGKS_NSTAGES closes the call).  Stages
 // a call skips are recorded at the same point, so they measure zero.
 static void mark(ggks_ctx* c, int stage) {
-  if (c->prof_cur < 0 || !((c->prof_mask >> stage) & 1u)) return;
+  if (c->prof_cur < 0 || !((c->prof_armed >> stage) & 1u)) return;
   const int per = GGKS_NSTAGES + 1;
   (void)hipEventRecord(c->prof[(size_t)c->prof_cur * per + stage], c->stream);
 }
@@ -509,6 +510,7 @@ int ggks_ctx_profile_arm(ggks_ctx* c, int nslots) {
     c->prof.push_back(e);
   }
   c->prof_slots = nslots;
+  c->prof_armed = c->prof_mask;
   return GGKS_OK;
 }
 
@@ -519,7 +521,7 @@ int ggks_ctx_profile_read(ggks_ctx* c, float* ms, int nslots) {
   HIP_TRY(hipSetDevice(c->device));
   HIP_TRY(hipStreamSynchronize(c->stream));
   const int per = GGKS_NSTAGES + 1;
-  const uint32_t mask = c->prof_mask;
+  const uint32_t mask = c->prof_armed;
   auto recorded = [&](int m) { return ((mask >> m) & 1u) != 0; };
   int first = -1, last = -1;
   for (int m = 0; m <= GGKS_NSTAGES; ++m)

@@ -149,7 +149,8 @@ int ggks_ctx_profile_read(ggks_ctx *ctx, float *ms, int nslots);
  * the two around the kernel it times.  ggks_ctx_profile_read() then reports
  * for a recorded event m the time to the next recorded one in ms[.. + m],
  * 0 for the others, and the first to the last recorded event as the whole
- * call.  Takes effect from the next select. */
+ * call.  Takes effect at the next ggks_ctx_profile_arm(), which latches the
+ * mask for the selects it times and for their read. */
 int ggks_ctx_profile_marks(ggks_ctx *ctx, uint32_t mask);
 
 /* ---- device plumbing for hosts without a GPU framework --------------

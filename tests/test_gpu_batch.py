@@ -77,6 +77,35 @@ def test_batch_many_children(pkg, O):
     check_outputs(b(d), expected(O, x, table, spec["k"]), table)
 
 
+def test_batch_large_child_small_buckets(pkg, O):
+    # segments whose mean bucket is small (the register-light finisher is
+    # chosen) but which hold a cluster of three adjacent values ~4000 times
+    # each: the bucket holding it is a child of ~12000 keys, past that
+    # finisher's registers, with up to four ranks inside it
+    W = pkg.workloads
+    n, nseg = 1 << 16, 4
+    xs, ks = [], []
+    for i in range(nseg):
+        x = W.uniform_f32(900 + i, n)
+        v = np.float32(0.125 * (i + 1))
+        cl = np.array([v, np.nextafter(v, np.float32(2)),
+                       np.nextafter(np.nextafter(v, np.float32(2)), np.float32(2))], np.float32)
+        pos = np.sort(W.k_uniform(950 + i, n, 12000, dedup=True).astype(np.int64) - 1)
+        x[pos] = cl[np.arange(pos.size) % 3]
+        s = np.sort(x)
+        first, last = np.searchsorted(s, cl[0]), np.searchsorted(s, cl[2], side="right")
+        k = n - np.array([first + 5, (first + last) // 2, last - 5, first + (last - first) // 3][:1 + i])
+        ks.append(np.concatenate([k, [1 + i * 1000]]).astype(np.uint32))
+        xs.append(x)
+    table = pkg.segment_table([n] * nseg, [0] * nseg, [k.size for k in ks])
+    x, k = np.concatenate(xs), np.concatenate(ks)
+    b = pkg.Batch(np.float32, table, k)
+    assert b.stats()["engine_segments"] == nseg
+    d = pkg.DeviceArray.from_host(x)
+    check_outputs(b(d), expected(O, x, table, k), table)
+    check_outputs(b(d, graph=True), expected(O, x, table, k), table)
+
+
 def test_batch_graph_replays(pkg, O):
     spec, table, x = make(pkg, np.float32, 11, 200, 8, 21)
     d = pkg.DeviceArray.from_host(x)
