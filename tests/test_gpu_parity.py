@@ -165,7 +165,9 @@ def test_async_entry_and_bad_k_status(pkg, O):
 def test_stage_events_and_marks(pkg, O):
     """Stage timing: all events give every stage and a whole-call time equal
     to their sum; a mask of the two events around the compaction pass gives
-    that stage alone (the others read 0) and the same outputs."""
+    that stage alone (the others read 0) and the same outputs.  The mask is
+    latched when profiling is armed: changing it before the read changes
+    nothing."""
     from ggks_multiselect_amd import _lib as L
     lib = L.lib()
     W = pkg.workloads
@@ -184,6 +186,7 @@ def test_stage_events_and_marks(pkg, O):
         for _ in range(2):
             pkg.multiselect_async(d, dk, dout, ctx)
         pkg.sync()
+        L.check(lib.ggks_ctx_profile_marks(ctx.handle, (1 << per) - 1 - mask | 1))
         ms = np.zeros(2 * per, np.float32)
         L.check(lib.ggks_ctx_profile_read(ctx.handle, ms.ctypes.data, 2))
         ms = ms.reshape(2, per)
