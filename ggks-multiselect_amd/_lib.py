@@ -24,6 +24,9 @@ EXPORTS = [
     "ggks_version", "ggks_bucket_multiselect", "ggks_bucket_multiselect_f32",
     "ggks_bucket_multiselect_f64", "ggks_bucket_multiselect_u32",
     "ggks_bucket_multiselect_i32", "ggks_radix_multiselect", "ggks_topk",
+    "ggks_radix_multiselect_f32", "ggks_radix_multiselect_f64", "ggks_radix_multiselect_u32",
+    "ggks_radix_multiselect_i32", "ggks_topk_f32", "ggks_topk_f64", "ggks_topk_u32",
+    "ggks_topk_i32",
     "ggks_bucket_multiselect_async", "ggks_radix_multiselect_async", "ggks_topk_async",
     "ggks_ctx_reserve", "ggks_ctx_check", "ggks_ctx_stream", "ggks_dev_alloc",
     "ggks_dev_free", "ggks_h2d", "ggks_d2h", "ggks_d2d", "ggks_sync",
@@ -36,6 +39,11 @@ EXPORTS = [
 ]
 BATCH_GRAPH = 1
 GEN_UNIFORM, GEN_NORMAL, GEN_LEVELS, GEN_CAUCHY, GEN_PLATEAU = 0, 1, 2, 3, 4
+GEN_SORTED, GEN_KILLER, GEN_NOISY, GEN_HALF_NORMAL, GEN_HUGE = 5, 6, 7, 8, 9
+GEN_ONES, GEN_ONES_TWOS, GEN_NORMAL100 = 10, 11, 12
+GEN_FAMILIES = {"uniform": 0, "normal": 1, "levels": 2, "cauchy": 3, "plateau": 4, "sorted": 5,
+                "killer": 6, "noisy": 7, "half_normal": 8, "huge": 9, "ones": 10,
+                "ones_twos": 11, "normal100": 12}
 NSTAGES = 8
 STAGES = ["queries", "sample", "lut", "count", "reduce", "findk", "compact", "finish"]
 
@@ -94,7 +102,7 @@ class ServerStats(ctypes.Structure):
     """ggks_server_stats (include/ggks.h)."""
     _fields_ = [(f, ctypes.c_uint64) for f in (
         "submitted", "completed", "rejected", "batches", "elements", "max_pending", "pending",
-        "scratch_budget", "scratch_in_use", "peak_scratch_bytes")]
+        "scratch_budget", "scratch_in_use", "peak_scratch_bytes", "failed")]
 
 
 _lib = None
@@ -162,6 +170,8 @@ def lib() -> ctypes.CDLL:
     }
     for t in ("f32", "f64", "u32", "i32"):
         sig[f"ggks_bucket_multiselect_{t}"] = ([vp, i64, vp, ctypes.c_int32, vp, optp], i32)
+        sig[f"ggks_radix_multiselect_{t}"] = ([vp, i64, vp, ctypes.c_int32, vp, optp], i32)
+        sig[f"ggks_topk_{t}"] = ([vp, i64, ctypes.c_int32, vp, optp], i32)
     for name, (args, res) in sig.items():
         f = getattr(L, name)
         f.argtypes = args

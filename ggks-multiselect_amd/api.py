@@ -308,13 +308,13 @@ def tuning_reset() -> None:
 def generate(dtype, family: str, seed: int, n: int, start: int = 0, lo: int = 0, hi: int = 0,
              out: DeviceArray | None = None, offset: int = 0):
     """Fill a device array (new, or `out` from element `offset`) with n seeded
-    values: family "uniform" (floats in [0,1), integers in [lo, hi]),
-    "normal", "levels" (floats holding the integers lo..hi), "cauchy"
-    (clipped to finite) or "plateau" (0.5 with probability 1/2, else
-    uniform)."""
+    values of a family of ggks.h's ggks_generate: "uniform" (floats in [0,1),
+    integers in [lo, hi]), "normal", "levels" (floats holding the integers
+    lo..hi), "cauchy", "plateau", "sorted" (positions start.. of a
+    non-decreasing sequence of length hi), "killer", "noisy", "half_normal",
+    "huge", "ones", "ones_twos" or "normal100"."""
     d = out if out is not None else DeviceArray(n, dtype)
-    fam = {"uniform": _lib.GEN_UNIFORM, "normal": _lib.GEN_NORMAL, "levels": _lib.GEN_LEVELS,
-           "cauchy": _lib.GEN_CAUCHY, "plateau": _lib.GEN_PLATEAU}[family]
+    fam = _lib.GEN_FAMILIES[family]
     if offset < 0 or offset + n > d.n:
         raise ValueError("generate range outside the array")
     check(lib().ggks_generate(d.ptr + offset * d.dtype.itemsize, dtype_code(d.dtype), fam,
@@ -342,12 +342,17 @@ class Server:
         check(lib().ggks_server_create(ctypes.byref(p), int(device), ptrs, lens,
                                        int(scratch_budget), int(max_inflight), int(max_batch)))
         self.handle = p.value
-        self._live = {}  # tag -> (k, out) kept alive until completion
+        # internal tag -> (caller's tag, k, out): the host buffers a pending
+        # request points at stay alive until it is reported; internal tags
+        # are unique, so callers may reuse or repeat their own tags
+        self._live = {}
+        self._next = 1
 
     def submit(self, dtype, offset: int, length: int, out: np.ndarray, k=None, K: int = 0,
                kind: str = "bucket", tag: int = 0) -> None:
         r = _lib.Request()
-        r.offset, r.length, r.tag = int(offset), int(length), int(tag)
+        itag = self._next
+        r.offset, r.length, r.tag = int(offset), int(length), itag
         r.dtype = dtype_code(dtype)
         if kind == "topk":
             r.kind, r.num_k, kk = _lib.KIND_TOPK, int(K), None
@@ -358,7 +363,8 @@ class Server:
             raise ValueError("output buffer must be a contiguous array of the dtype, num_k long")
         r.h_out = out.ctypes.data
         check(lib().ggks_server_submit(self.handle, ctypes.byref(r)))
-        self._live[int(tag)] = (kk, out)
+        self._next += 1
+        self._live[itag] = (int(tag), kk, out)
 
     def poll(self, max_done: int = 4096) -> list:
         tags = np.zeros(max_done, np.uint64)
@@ -366,9 +372,11 @@ class Server:
         n = lib().ggks_server_poll(self.handle, tags.ctypes.data, lat.ctypes.data, max_done)
         if n < 0:
             check(n)
-        for t in tags[:n]:
-            self._live.pop(int(t), None)
-        return list(zip(tags[:n].tolist(), lat[:n].tolist()))
+        done = []
+        for t, l in zip(tags[:n].tolist(), lat[:n].tolist()):
+            user_tag = self._live.pop(int(t))[0]
+            done.append((user_tag, l))
+        return done
 
     def drain(self) -> None:
         check(lib().ggks_server_drain(self.handle))

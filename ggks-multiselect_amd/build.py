@@ -16,7 +16,7 @@ ARCH = "gfx950"
 SOURCES = [os.path.join(HERE, "csrc", "ggks_select.hip")]
 HEADERS = [os.path.join(HERE, "csrc", h) for h in
            ("ggks_keys.hpp", "ggks_types.hpp", "ggks_kernels.hpp", "ggks_plan.hpp",
-            "ggks_batch.hpp", "ggks_serve.hpp", "ggks_gen.hpp")]
+            "ggks_host.hpp", "ggks_batch.hpp", "ggks_serve.hpp", "ggks_gen.hpp")]
 OUT = os.path.join(HERE, "libggks.so")
 # C++ harness of the reference-named wrappers (include/ggks_compat.hpp):
 # host-only g++, linked against libggks.so

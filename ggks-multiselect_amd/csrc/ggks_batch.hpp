@@ -130,6 +130,11 @@ static int batch_create(ggks_batch** out, int device, int dtype, const ggks_segm
   if (rc) return rc;
   ggks_opts o = {};
   if (opt) o = *opt;
+  {
+    Plan probe;
+    if (!plan_batch(probe, dtype, in, o))
+      return fail(GGKS_EINVAL, "batch too large for one plan (32-bit table offsets); split it");
+  }
   HIP_TRY(hipSetDevice(device));
   ggks_batch* b = new ggks_batch();
   b->device = device;

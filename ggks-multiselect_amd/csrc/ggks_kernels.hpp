@@ -1083,8 +1083,13 @@ __device__ __forceinline__ void compact_chunk(const KT* __restrict__ src,
   const uint8_t* const act_of = reinterpret_cast<const uint8_t*>(abyte);
   uint32_t* row = rows + sd.row_base + (uint64_t)ch.idx * sd.bmax;
   const Child* kids = children + sd.q_off;
+  // Cursors are positions inside the segment's part of the reduced buffer
+  // (child start - red_base < segment length < 2^32), so they stay 32-bit
+  // however far into the buffer the segment sits (batches and served groups
+  // of more than 2^32 keys).
+  KT* const rseg = red + sd.red_base;
   for (uint32_t u = tid; u < U && u < kCompactCursors; u += kThreads)
-    cur[u] = (uint32_t)kids[u].start + row[kids[u].bucket];
+    cur[u] = (uint32_t)(kids[u].start - sd.red_base) + row[kids[u].bucket];
   if (tid == 0) nbuf = 0;
   __syncthreads();
   BucketFn<KT> bk;
@@ -1098,8 +1103,9 @@ __device__ __forceinline__ void compact_chunk(const KT* __restrict__ src,
       const uint32_t u = wbase[b >> 5] + __popc(bits & ((1u << sh) - 1u));
       const uint32_t pos = u < kCompactCursors
                                ? atomicAdd(&cur[u], 1u)
-                               : (uint32_t)kids[u].start + atomicAdd(&row[kids[u].bucket], 1u);
-      red[pos] = raw;
+                               : (uint32_t)(kids[u].start - sd.red_base) +
+                                     atomicAdd(&row[kids[u].bucket], 1u);
+      rseg[pos] = raw;
     }
   };
   // every thread, with the same n: place the buffered keys, empty the buffer.
@@ -1122,8 +1128,9 @@ __device__ __forceinline__ void compact_chunk(const KT* __restrict__ src,
       for (int f = 0; f < F; ++f) {
         const uint32_t pos = u[f] < kCompactCursors
                                  ? atomicAdd(&cur[u[f]], 1u)
-                                 : (uint32_t)kids[u[f]].start + atomicAdd(&row[kids[u[f]].bucket], 1u);
-        red[pos] = kk[f];
+                                 : (uint32_t)(kids[u[f]].start - sd.red_base) +
+                                       atomicAdd(&row[kids[u[f]].bucket], 1u);
+        rseg[pos] = kk[f];
       }
     }
     for (; i < n; i += kThreads) place(kb[i]);

@@ -33,6 +33,13 @@ inline Tuning default_tuning(uint64_t n, int eb) {
   return {10, 32};
 }
 
+// Bucket budget (log2) of a radix segment with digits of rb bits: room for
+// the 2^rb digits plus the edge buckets, and never below 2^8, so that every
+// segment's bucket tables are whole 16-byte words of group sums (the bucket
+// prefix kernel loads bmax / 256 of them) and every segment's table offsets
+// in a batch stay multiples of 256 buckets.
+inline uint32_t radix_budget_log2(uint32_t rb) { return std::max<uint32_t>(rb + 1, 8); }
+
 struct Plan {
   std::vector<SegDesc> segs;
   std::vector<Chunk> chunks;
@@ -113,7 +120,7 @@ inline void plan_single(Plan& p, int kind, int dtype, uint64_t n, uint32_t nq,
     mode = MODE_RADIX;
     rb = o.radix_bits ? (uint32_t)o.radix_bits : 14u;
     rb = std::min<uint32_t>(14, std::max<uint32_t>(4, rb));
-    bl = rb + 1;
+    bl = radix_budget_log2(rb);
     pb = std::min<uint32_t>(kMaxPrefixBits, rb);
   } else {
     bl = o.buckets_log2 ? (uint32_t)o.buckets_log2 : t.buckets_log2;
@@ -249,17 +256,28 @@ struct BatchSeg {
   uint64_t out_offset;
 };
 
+// Largest value of the 32-bit table offsets a batch plan may reach (segment
+// descriptors hold sample, LUT, bucket-table, bitmap and query offsets as
+// uint32; kernels index chunks, items and sample blocks by 32-bit grid
+// indices).  Element offsets (segment data, reduced buffer, rows) are 64-bit
+// and unbounded here.
+constexpr uint64_t kPlanTableLimit = 0xFFFFFFFFull;
+constexpr uint64_t kPlanGridLimit = 0x7FFFFFFFull;
+
 // Plan a batch: segments whose band is marked direct (and whose length fits
 // in the finisher's LDS) become direct children with explicit queries; the
 // rest become engine segments with their own chunk rows, LUT, bitmaps and a
-// slice of the reduced buffer.
-inline void plan_batch(Plan& p, int dtype, const std::vector<BatchSeg>& in, const ggks_opts& o) {
+// slice of the reduced buffer.  Returns false (and leaves p unusable) when
+// the batch is too large for the 32-bit table offsets: the caller refuses it
+// or splits it.
+inline bool plan_batch(Plan& p, int dtype, const std::vector<BatchSeg>& in, const ggks_opts& o) {
   const int eb = dtype_bytes(dtype);
   const uint64_t cap = kFinishLdsBytes / (uint32_t)eb;
   const TuningTable& tab = tuning_table();
   p.seed = o.seed ? o.seed : 0x6A09E667F3BCC909ull;
   uint64_t red = 0, rowc = 0;
-  uint32_t qoff = 0, sam = 0, lutc = 0, totc = 0, bitc = 0, dq = 0;
+  // 64-bit running totals of the 32-bit offsets, checked before each use
+  uint64_t qoff = 0, sam = 0, lutc = 0, totc = 0, bitc = 0, dq = 0;
   for (size_t i = 0; i < in.size(); ++i) {
     const BatchSeg& g = in[i];
     if (g.num_k == 0) continue;
@@ -267,10 +285,12 @@ inline void plan_batch(Plan& p, int dtype, const std::vector<BatchSeg>& in, cons
     const BandTuning bt = tab.t[dtype][band_of(n)];
     const bool topk = g.kind == GGKS_KIND_TOPK;
     if (n <= cap && (bt.direct || n <= 64)) {
+      if (dq + g.num_k > kPlanTableLimit || p.direct_children.size() >= kPlanGridLimit)
+        return false;
       Child c = {};
       c.start = g.offset;
       c.len = n;
-      c.q_off = dq;
+      c.q_off = (uint32_t)dq;
       c.q_cnt = g.num_k;
       c.src = 0;
       c.need_range = 1;
@@ -290,7 +310,7 @@ inline void plan_batch(Plan& p, int dtype, const std::vector<BatchSeg>& in, cons
       mode = MODE_RADIX;
       rb = o.radix_bits ? (uint32_t)o.radix_bits : 14u;
       rb = std::min<uint32_t>(14, std::max<uint32_t>(4, rb));
-      bl = rb + 1;
+      bl = radix_budget_log2(rb);
       pb = std::min<uint32_t>(kMaxPrefixBits, rb);
     } else {
       bl = o.buckets_log2 ? (uint32_t)o.buckets_log2 : (bt.buckets_log2 ? bt.buckets_log2 : 12u);
@@ -312,8 +332,16 @@ inline void plan_batch(Plan& p, int dtype, const std::vector<BatchSeg>& in, cons
     if (mode == MODE_SAMPLE) {
       uint64_t want = (n / 1024 + 1023) / 1024 * 1024;
       ns = (uint32_t)std::min<uint64_t>(65536, std::max<uint64_t>(1024, want));
-      for (uint32_t y = 0; y < ns / kThreads; ++y) p.sblocks.push_back({(uint32_t)p.segs.size(), y});
     }
+    // every 32-bit offset this segment would take, and the grids, must fit
+    const uint32_t nitems = (((bmax + kReduceBins - 1) / kReduceBins) >> reduce_groups_log2(G)) + 1;
+    if (qoff + g.num_k > kPlanTableLimit || sam + ns > kPlanTableLimit ||
+        lutc + (1u << pb) > kPlanTableLimit || totc + bmax > kPlanTableLimit ||
+        bitc + bmax / 32 > kPlanTableLimit || p.segs.size() + 1 > kPlanGridLimit ||
+        p.chunks.size() + G > kPlanGridLimit || p.items.size() + nitems > kPlanGridLimit ||
+        p.sblocks.size() + ns / kThreads > kPlanGridLimit)
+      return false;
+    for (uint32_t y = 0; y < ns / kThreads; ++y) p.sblocks.push_back({(uint32_t)p.segs.size(), y});
     SegDesc s = {};
     s.data_off = g.offset;
     s.len = n;
@@ -322,12 +350,12 @@ inline void plan_batch(Plan& p, int dtype, const std::vector<BatchSeg>& in, cons
     s.bmax = bmax;
     s.pbits = pb;
     s.nchunks = (uint32_t)G;
-    s.sample_off = sam;
+    s.sample_off = (uint32_t)sam;
     s.nsample = ns;
-    s.lut_off = lutc;
-    s.tot_off = totc;
-    s.bits_off = bitc;
-    s.q_off = qoff;
+    s.lut_off = (uint32_t)lutc;
+    s.tot_off = (uint32_t)totc;
+    s.bits_off = (uint32_t)bitc;
+    s.q_off = (uint32_t)qoff;
     s.q_cnt = g.num_k;
     s.mode = mode;
     s.radix_bits = rb;
@@ -359,7 +387,7 @@ inline void plan_batch(Plan& p, int dtype, const std::vector<BatchSeg>& in, cons
     totc += bmax;
     bitc += (bmax + 31) / 32;
   }
-  p.nq = qoff;
+  p.nq = (uint32_t)qoff;
   size_t off = 0;
   auto take = [&](size_t bytes) {
     size_t at = off;
@@ -377,8 +405,8 @@ inline void plan_batch(Plan& p, int dtype, const std::vector<BatchSeg>& in, cons
   p.off_dqueries = take(p.direct_queries.size() * sizeof(Query));
   p.tables_end = off;
   p.off_states = take(nseg * sizeof(SegState));
-  p.off_samples = take((size_t)std::max<uint32_t>(sam, 1) * eb);
-  p.off_minmax = take((size_t)std::max<uint32_t>(sam / 1024, 1) * 2 * eb);
+  p.off_samples = take((size_t)std::max<uint64_t>(sam, 1) * eb);
+  p.off_minmax = take((size_t)std::max<uint64_t>(sam / 1024, 1) * 2 * eb);
   p.off_lut = take((size_t)lutc * 4);
   p.off_rows = take((size_t)rowc * 4);
   p.off_tot = take((size_t)totc * 4);
@@ -392,6 +420,7 @@ inline void plan_batch(Plan& p, int dtype, const std::vector<BatchSeg>& in, cons
   p.off_cq = take((size_t)qoff * sizeof(Query));
   p.off_red = take((size_t)red * eb);
   p.total = off;
+  return true;
 }
 
 }  // namespace ggks

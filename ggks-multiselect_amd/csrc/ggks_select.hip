@@ -20,6 +20,7 @@
 #include "ggks.h"
 #include "ggks_kernels.hpp"
 #include "ggks_plan.hpp"
+#include "ggks_host.hpp"
 #include "ggks_gen.hpp"
 
 namespace ggks {
@@ -631,6 +632,28 @@ GGKS_TYPED(ggks_bucket_multiselect_f32, float, GGKS_F32)
 GGKS_TYPED(ggks_bucket_multiselect_f64, double, GGKS_F64)
 GGKS_TYPED(ggks_bucket_multiselect_u32, uint32_t, GGKS_U32)
 GGKS_TYPED(ggks_bucket_multiselect_i32, int32_t, GGKS_I32)
+#undef GGKS_TYPED
+
+#define GGKS_TYPED_RADIX(NAME, T, DT)                                                    \
+  int NAME(const T* d_vec, int64_t n, const uint32_t* h_k, int32_t num_k, T* h_out,      \
+           const ggks_opts* opt) {                                                       \
+    return ggks_radix_multiselect(nullptr, DT, d_vec, n, h_k, num_k, h_out, opt);        \
+  }
+GGKS_TYPED_RADIX(ggks_radix_multiselect_f32, float, GGKS_F32)
+GGKS_TYPED_RADIX(ggks_radix_multiselect_f64, double, GGKS_F64)
+GGKS_TYPED_RADIX(ggks_radix_multiselect_u32, uint32_t, GGKS_U32)
+GGKS_TYPED_RADIX(ggks_radix_multiselect_i32, int32_t, GGKS_I32)
+#undef GGKS_TYPED_RADIX
+
+#define GGKS_TYPED_TOPK(NAME, T, DT)                                                      \
+  int NAME(const T* d_vec, int64_t n, int32_t K, T* h_out, const ggks_opts* opt) {       \
+    return ggks_topk(nullptr, DT, d_vec, n, K, h_out, opt);                               \
+  }
+GGKS_TYPED_TOPK(ggks_topk_f32, float, GGKS_F32)
+GGKS_TYPED_TOPK(ggks_topk_f64, double, GGKS_F64)
+GGKS_TYPED_TOPK(ggks_topk_u32, uint32_t, GGKS_U32)
+GGKS_TYPED_TOPK(ggks_topk_i32, int32_t, GGKS_I32)
+#undef GGKS_TYPED_TOPK
 
 // ---------------------------------------------------------- plumbing
 int ggks_dev_alloc(void** ptr, size_t bytes) {
@@ -705,7 +728,10 @@ size_t ggks_batch_scratch_bytes(int dtype, const ggks_segment* segs, int64_t num
   ggks_opts o = {};
   if (opt) o = *opt;
   Plan p;
-  plan_batch(p, dtype, in, o);
+  if (!plan_batch(p, dtype, in, o)) {
+    fail(GGKS_EINVAL, "batch too large for one plan (32-bit table offsets); split it");
+    return 0;
+  }
   return batch_arena_bytes(p);
 }
 
@@ -764,16 +790,23 @@ int ggks_tuning_reset(void) {
 int ggks_generate(void* d_ptr, int dtype, int family, uint64_t seed, uint64_t start, uint64_t n,
                   int64_t lo, int64_t hi) {
   if (!d_ptr && n) return fail(GGKS_EINVAL, "null pointer");
-  if (dtype < 0 || dtype > 3 || family < 0 || family > 4)
+  if (dtype < 0 || dtype > 3 || family < 0 || family >= GEN_FAMILIES)
     return fail(GGKS_EINVAL, "bad dtype %d or family %d", dtype, family);
-  if (family >= 3 && (dtype == GGKS_U32 || dtype == GGKS_I32))
+  const bool integer = dtype == GGKS_U32 || dtype == GGKS_I32;
+  if (integer && family != GEN_UNIFORM && family != GEN_LEVELS && family != GEN_SORTED &&
+      family != GEN_ONES && family != GEN_ONES_TWOS)
     return fail(GGKS_EINVAL, "family %d is for float dtypes", family);
-  if (family == 2 && (hi < lo || lo < 0 || hi > 0xFFFFFFFFll))
+  if (family == GEN_SORTED) {
+    if (hi < 1 || start > (uint64_t)hi || n > (uint64_t)hi - start)
+      return fail(GGKS_EINVAL, "sorted: positions [%llu, +%llu) outside the length %lld",
+                  (unsigned long long)start, (unsigned long long)n, (long long)hi);
+  } else if (family == GEN_LEVELS && !integer && (hi < lo || lo < 0 || hi > 0xFFFFFFFFll)) {
     return fail(GGKS_EINVAL, "bad levels range [%lld, %lld]", (long long)lo, (long long)hi);
-  if ((dtype == GGKS_U32 || dtype == GGKS_I32) &&
-      (hi < lo || (dtype == GGKS_U32 && (lo < 0 || hi > 0xFFFFFFFFll)) ||
-       (dtype == GGKS_I32 && (lo < INT32_MIN || hi > INT32_MAX))))
+  } else if (integer && (family == GEN_UNIFORM || family == GEN_LEVELS) &&
+             (hi < lo || (dtype == GGKS_U32 && (lo < 0 || hi > 0xFFFFFFFFll)) ||
+              (dtype == GGKS_I32 && (lo < INT32_MIN || hi > INT32_MAX)))) {
     return fail(GGKS_EINVAL, "bad integer range [%lld, %lld]", (long long)lo, (long long)hi);
+  }
   if (n == 0) return GGKS_OK;
   const uint32_t grid = (uint32_t)std::min<uint64_t>((n + 255) / 256, 65536);
   hipLaunchKernelGGL(k_generate, dim3(grid), dim3(256), 0, nullptr, d_ptr, dtype, family, seed,

@@ -23,58 +23,6 @@ inline double now_s() {
       .count();
 }
 
-// First-fit allocator over [0, size) with 256-byte granules.
-class RangeAlloc {
- public:
-  void reset(size_t size) {
-    free_.clear();
-    if (size) free_[0] = size;
-    used_ = 0;
-  }
-  bool alloc(size_t bytes, size_t& off) {
-    bytes = align_up(bytes);
-    for (auto it = free_.begin(); it != free_.end(); ++it) {
-      if (it->second < bytes) continue;
-      off = it->first;
-      const size_t rest = it->second - bytes;
-      free_.erase(it);
-      if (rest) free_[off + bytes] = rest;
-      used_ += bytes;
-      return true;
-    }
-    return false;
-  }
-  void release(size_t off, size_t bytes) {
-    bytes = align_up(bytes);
-    used_ -= bytes;
-    auto it = free_.emplace(off, bytes).first;
-    auto nx = std::next(it);
-    if (nx != free_.end() && it->first + it->second == nx->first) {
-      it->second += nx->second;
-      free_.erase(nx);
-    }
-    if (it != free_.begin()) {
-      auto pv = std::prev(it);
-      if (pv->first + pv->second == it->first) {
-        pv->second += it->second;
-        free_.erase(it);
-      }
-    }
-  }
-  size_t used() const { return used_; }
-
- private:
-  std::map<size_t, size_t> free_;
-  size_t used_ = 0;
-};
-
-struct ServeReq {
-  ggks_request r;
-  std::vector<uint32_t> k;
-  double submitted = 0;
-  int64_t trace_idx = -1;  // position in a run_trace request list
-};
-
 struct ServeSlot {
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
@@ -82,7 +30,8 @@ struct ServeSlot {
   size_t off = 0, bytes = 0;  // scratch sub-allocation
   char* pinned = nullptr;
   size_t pinned_cap = 0;
-  size_t pin_out = 0;  // outputs inside the pinned buffer
+  size_t pin_stat = 0;  // the batch's status block inside the pinned buffer
+  size_t pin_out = 0;   // outputs inside the pinned buffer
   std::vector<size_t> ids;
   std::vector<uint64_t> out_at;  // per request: first output slot in the batch
   int dtype = 0;
@@ -99,22 +48,16 @@ struct ggks_server {
   ggks::RangeAlloc alloc;
   std::vector<ggks::ServeSlot> slots;
   std::deque<size_t> pending;
-  std::vector<ggks::ServeReq> reqs;  // by internal id
-  std::deque<std::pair<uint64_t, double>> finished;  // (tag, latency)
+  std::vector<ggks::ServeReq> reqs;  // by internal id (slots are recycled)
+  std::vector<size_t> free_ids;      // request slots free for reuse
+  std::deque<std::pair<uint64_t, double>> finished;  // (tag, latency; -1: failed)
   uint32_t max_batch = 256;
   ggks_server_stats st = {};
   double* trace_latency = nullptr;  // run_trace: latency per request id
+  size_t last_id = 0;               // slot of the last submitted request
 };
 
 namespace ggks {
-
-static size_t serve_footprint(int dtype, const std::vector<BatchSeg>& segs, Plan& p) {
-  ggks_opts o = {};
-  plan_batch(p, dtype, segs, o);
-  uint64_t outs = 0;
-  for (const BatchSeg& g : segs) outs += g.num_k;
-  return align_up(p.total) + 256 + align_up((size_t)outs * dtype_bytes(dtype));
-}
 
 static int serve_validate(const ggks_server* s, const ggks_request* r) {
   if (!r) return fail(GGKS_EINVAL, "null request");
@@ -140,17 +83,6 @@ static int serve_validate(const ggks_server* s, const ggks_request* r) {
   return GGKS_OK;
 }
 
-static BatchSeg serve_seg(const ServeReq& q, uint64_t out_at) {
-  BatchSeg g = {};
-  g.offset = q.r.offset;
-  g.length = q.r.length;
-  g.k = q.r.kind == GGKS_KIND_TOPK ? nullptr : q.k.data();
-  g.num_k = q.r.num_k;
-  g.kind = (uint32_t)q.r.kind;
-  g.out_offset = out_at;
-  return g;
-}
-
 template <typename KT, int C>
 static int serve_enqueue_typed(const Exec& x, const Plan& p, const void* src, void* out,
                                const uint32_t* d_k) {
@@ -170,7 +102,10 @@ static int serve_launch(ggks_server* s, ServeSlot& sl, const std::vector<size_t>
     outs += s->reqs[id].r.num_k;
   }
   const size_t tb = p.table_bytes();
-  const size_t pin_out = align_up(tb);
+  // pinned layout: tables | status block (256 B) | outputs; the status block
+  // and the outputs come back in one copy (they are adjacent on the device)
+  const size_t pin_stat = align_up(tb);
+  const size_t pin_out = pin_stat + 256;
   const size_t pin_need = pin_out + (size_t)outs * eb;
   if (pin_need > sl.pinned_cap) {
     if (sl.pinned) HIP_TRY(hipHostFree(sl.pinned));
@@ -201,12 +136,13 @@ static int serve_launch(ggks_server* s, ServeSlot& sl, const std::vector<size_t>
     default: rc = serve_enqueue_typed<uint32_t, CONV_I>(x, p, src, d_out, d_k); break;
   }
   if (rc) return rc;
-  HIP_TRY(hipMemcpyAsync(sl.pinned + pin_out, d_out, (size_t)outs * eb, hipMemcpyDeviceToHost,
-                         sl.stream));
+  HIP_TRY(hipMemcpyAsync(sl.pinned + pin_stat, status, 256 + (size_t)outs * eb,
+                         hipMemcpyDeviceToHost, sl.stream));
   HIP_TRY(hipEventRecord(sl.done, sl.stream));
   sl.busy = true;
   sl.off = off;
   sl.bytes = foot;
+  sl.pin_stat = pin_stat;
   sl.pin_out = pin_out;
   sl.ids = ids;
   sl.dtype = dtype;
@@ -215,50 +151,24 @@ static int serve_launch(ggks_server* s, ServeSlot& sl, const std::vector<size_t>
   return GGKS_OK;
 }
 
-// Form and launch batches while slots and scratch allow.
+// Form and launch batches while slots and scratch allow (admission:
+// serve_form_group / serve_admit).
 static int serve_schedule(ggks_server* s) {
   for (ServeSlot& sl : s->slots) {
     if (sl.busy) continue;
     if (s->pending.empty()) return GGKS_OK;
-    const int dtype = s->reqs[s->pending.front()].r.dtype;
-    std::vector<size_t> group;
-    for (size_t id : s->pending) {
-      if (s->reqs[id].r.dtype != dtype) continue;
-      group.push_back(id);
-      if (group.size() >= s->max_batch) break;
+    const std::vector<size_t> group = serve_form_group(s->reqs, s->pending, s->max_batch);
+    Plan p;
+    size_t foot = 0, off = 0;
+    const size_t take = serve_admit(s->alloc, s->reqs, group, p, foot, off);
+    if (take == 0) return GGKS_OK;  // wait for memory to free up
+    std::vector<size_t> ids(group.begin(), group.begin() + take);
+    int rc = serve_launch(s, sl, ids, p, foot, off);
+    if (rc) {
+      s->alloc.release(off, foot);
+      return rc;
     }
-    // admission: the largest prefix of the group (halving) whose scratch fits
-    size_t take = group.size();
-    for (;;) {
-      std::vector<BatchSeg> segs;
-      uint64_t at = 0;
-      for (size_t i = 0; i < take; ++i) {
-        segs.push_back(serve_seg(s->reqs[group[i]], at));
-        at += s->reqs[group[i]].r.num_k;
-      }
-      Plan p;
-      const size_t foot = serve_footprint(dtype, segs, p);
-      size_t off = 0;
-      if (s->alloc.alloc(foot, off)) {
-        std::vector<size_t> ids(group.begin(), group.begin() + take);
-        int rc = serve_launch(s, sl, ids, p, foot, off);
-        if (rc) {
-          s->alloc.release(off, foot);
-          return rc;
-        }
-        // drop the launched requests from the pending queue (order kept)
-        std::deque<size_t> rest;
-        size_t j = 0;
-        for (size_t id : s->pending) {
-          if (j < ids.size() && id == ids[j]) { ++j; continue; }
-          rest.push_back(id);
-        }
-        s->pending.swap(rest);
-        break;
-      }
-      if (take == 1) return GGKS_OK;  // wait for memory to free up
-      take = (take + 1) / 2;
-    }
+    serve_remove_launched(s->pending, ids);
   }
   return GGKS_OK;
 }
@@ -273,20 +183,35 @@ static int serve_harvest(ggks_server* s, size_t& ncompleted) {
     if (q != hipSuccess) return fail(GGKS_EDEVICE, "batch failed: %s", hipGetErrorString(q));
     const double t = now_s();
     const size_t eb = (size_t)dtype_bytes(sl.dtype);
+    // a status bit set by a kernel of this batch (requests are validated on
+    // submission, so only an internal capacity error can set one): its
+    // outputs are not trusted, each request is reported failed (latency -1,
+    // output untouched) and the poll / drain / trace call returns an error
+    uint32_t status = 0;
+    memcpy(&status, sl.pinned + sl.pin_stat, 4);
     for (size_t i = 0; i < sl.ids.size(); ++i) {
       ServeReq& rq = s->reqs[sl.ids[i]];
-      memcpy(rq.r.h_out, sl.pinned + sl.pin_out + sl.out_at[i] * eb, (size_t)rq.r.num_k * eb);
-      const double lat = t - rq.submitted;
+      double lat = -1.0;
+      if (!status) {
+        memcpy(rq.r.h_out, sl.pinned + sl.pin_out + sl.out_at[i] * eb, (size_t)rq.r.num_k * eb);
+        lat = t - rq.submitted;
+        s->st.completed++;
+        s->st.elements += rq.r.length;
+      } else {
+        s->st.failed++;
+      }
       if (s->trace_latency && rq.trace_idx >= 0) s->trace_latency[rq.trace_idx] = lat;
       else s->finished.emplace_back(rq.r.tag, lat);
-      s->st.completed++;
-      s->st.elements += rq.r.length;
       std::vector<uint32_t>().swap(rq.k);
+      rq.trace_idx = -1;
+      s->free_ids.push_back(sl.ids[i]);  // the request slot is reused
     }
     ncompleted += sl.ids.size();
     s->alloc.release(sl.off, sl.bytes);
     sl.busy = false;
     sl.ids.clear();
+    if (status)
+      return fail(GGKS_EDEVICE, "a served batch reported internal status 0x%x", status);
   }
   return GGKS_OK;
 }
@@ -308,8 +233,17 @@ static int serve_submit(ggks_server* s, const ggks_request* r, double at) {
     s->st.rejected++;
     return fail(GGKS_ENOMEM, "request needs more scratch than the %zu-byte budget", s->budget);
   }
-  s->reqs.push_back(std::move(q));
-  s->pending.push_back(s->reqs.size() - 1);
+  size_t id;
+  if (!s->free_ids.empty()) {  // reuse a finished request's slot
+    id = s->free_ids.back();
+    s->free_ids.pop_back();
+    s->reqs[id] = std::move(q);
+  } else {
+    id = s->reqs.size();
+    s->reqs.push_back(std::move(q));
+  }
+  s->pending.push_back(id);
+  s->last_id = id;
   s->st.submitted++;
   s->st.max_pending = std::max<uint64_t>(s->st.max_pending, s->pending.size());
   return GGKS_OK;
@@ -357,7 +291,7 @@ static int serve_run_trace(ggks_server* s, const ggks_request* reqs, const doubl
   while (i < n || !serve_idle(s)) {
     const double t = now_s() - t0;
     while (i < n && arrival[i] <= t) {
-      if (serve_submit(s, &reqs[i], t0 + arrival[i]) == GGKS_OK) s->reqs.back().trace_idx = i;
+      if (serve_submit(s, &reqs[i], t0 + arrival[i]) == GGKS_OK) s->reqs[s->last_id].trace_idx = i;
       else latency[i] = -1.0;
       ++i;
     }

@@ -110,6 +110,62 @@ def permutation_u32(seed: int, n: int) -> np.ndarray:
     return (np.argsort(keys, kind="stable") + 1).astype(np.uint32)
 
 
+def sorted_u32(seed: int, n: int, start: int = 0, total: int | None = None) -> np.ndarray:
+    """Positions start..start+n-1 of the non-decreasing sequence of length
+    `total` (default start + n): floor((j 2^32 + r_j) / total), r_j = the top
+    32 bits of SplitMix64 output j (the device family "sorted"; stands in for
+    generateProblems.cu:34-45, which sorts random uints)."""
+    total = start + n if total is None else total
+    assert 1 <= total < 2 ** 32 and start + n <= total
+
+    def gen(s, c):
+        j = np.arange(start + s, start + s + c, dtype=np.uint64)
+        r = splitmix64(seed, start + s, c) >> np.uint64(32)
+        return ((j << np.uint64(32)) | r) // np.uint64(total)
+    return _chunked(n, lambda s, c: gen(s, c).astype(np.uint32), np.uint32)
+
+
+def sorted_as(dtype, u: np.ndarray) -> np.ndarray:
+    """The "sorted" family in another dtype, from its uint32 values."""
+    d = np.dtype(dtype)
+    if d == np.uint32:
+        return u
+    if d == np.int32:
+        return (u ^ np.uint32(0x80000000)).view(np.int32)
+    v = u.astype(np.float64) * 2.0 ** -32
+    return v.astype(np.float32) if d == np.float32 else v
+
+
+def killer_f32(seed: int, n: int) -> np.ndarray:
+    """Bucket killer (generateProblems.cu:344-367): uniform floats with bits
+    masked by 0x80EFFFFF (tiny magnitudes) and positions 0..252 holding the
+    powers of two 2^-126..2^126 (unsorted; no result depends on order)."""
+    u = uniform_f32(seed, n).view(np.uint32) & np.uint32(0x80EFFFFF)
+    m = min(n, 253)
+    u[:m] = (np.arange(m, dtype=np.uint32) + np.uint32(1)) << np.uint32(23)
+    return u.view(np.float32)
+
+
+def killer_f64(seed: int, n: int) -> np.ndarray:
+    """Bucket killer doubles (generateProblems.cu:385-423): bits masked by
+    0x800FFFFFFFFFFFFF, positions 0..2044 holding 2^-1022..2^1022."""
+    u = uniform_f64(seed, n).view(np.uint64) & np.uint64(0x800FFFFFFFFFFFFF)
+    m = min(n, 2045)
+    u[:m] = (np.arange(m, dtype=np.uint64) + np.uint64(1)) << np.uint64(52)
+    return u.view(np.float64)
+
+
+def huge_f32(seed: int, n: int) -> np.ndarray:
+    """U[0,1) * 10^6 in float32 arithmetic (generateProblems.cu:287-298)."""
+    return uniform_f32(seed, n) * np.float32(1.0e6)
+
+
+def ones_twos(seed: int, n: int, dtype=np.float32) -> np.ndarray:
+    """1 or 2 with probability 1/2 each (generateProblems.cu:154-177 shape)."""
+    return _chunked(n, lambda s, c: (np.uint64(1) + (splitmix64(seed, s, c) >> np.uint64(63)))
+                    .astype(dtype), dtype)
+
+
 # ---------------------------------------------------------------- k sets
 def k_percentiles(n: int) -> np.ndarray:
     """101 ranks k = round(p (n-1)) + 1, p = 0.00 .. 1.00 (p = 0 is the max)."""
@@ -129,6 +185,55 @@ def k_cluster(seed: int, n: int, count: int = 256, width: int = 1000) -> np.ndar
     keys = splitmix64(seed, 0, width)
     pick = np.sort(np.argsort(keys, kind="stable")[:count]) + lo
     return np.concatenate([pick.astype(np.uint32), np.array([1, n], np.uint32)])
+
+
+def _unit_f32(seed: int, count: int) -> np.ndarray:
+    """float32 uniforms in [0,1) on the 24-bit grid (stand-in for
+    curandGenerateUniform in the reference's k generators)."""
+    return (splitmix64(seed, 0, count) >> np.uint64(40)).astype(np.float32) * np.float32(2.0 ** -24)
+
+
+def k_even(count: int, n: int) -> np.ndarray:
+    """"Uniform Ks" (generateProblems.cu:506-513): k_0 = 1, k_i =
+    (uint)((i / (float)count) * n) in float32 arithmetic, k_{count-1} = n."""
+    i = np.arange(count, dtype=np.float32)
+    k = ((i / np.float32(count)) * np.float32(n)).astype(np.uint32)
+    k[0] = 1
+    k[-1] = n
+    return np.clip(k, 1, n).astype(np.uint32)
+
+
+def k_normal(seed: int, count: int, n: int) -> np.ndarray:
+    """"Normal Random Ks" (generateProblems.cu:515-531): (|z| + 7) * (n / 14)
+    with z ~ N(0,1); the reference can exceed n for |z| > 7, clamped here."""
+    z = _normal64(seed, 0, count)
+    k = ((np.abs(z) + 7.0) * (n / 14.0)).astype(np.uint64)
+    return np.clip(k, 1, n).astype(np.uint32)
+
+
+def k_cluster9(seed: int, count: int, n: int) -> np.ndarray:
+    """"Cluster Ks" (generateProblems.cu:533-557): clusters of 9 consecutive
+    ranks c-4..c+4 around centres c = (uint)(u (n - 10) + 5).  The
+    reference leaves the last count mod 9 entries unwritten when count is
+    not a multiple of 9; here they continue the last cluster upwards
+    (clamped to n)."""
+    assert n >= 10
+    nc = max(1, (count + 8) // 9)
+    u = _unit_f32(seed, nc)
+    centres = (u * np.float32(n - 10) + np.float32(5)).astype(np.int64)
+    k = centres[:, None] + np.arange(-4, 5, dtype=np.int64)[None, :]
+    return np.clip(k.reshape(-1)[:count], 1, n).astype(np.uint32)
+
+
+def k_sectioned(seed: int, count: int, n: int) -> np.ndarray:
+    """"Sectioned Ks" (generateProblems.cu:559-576): count consecutive ranks
+    from k_0 = (uint)(u (n - count)), kept >= 1 here (the reference can
+    start at 0, an invalid rank)."""
+    assert count <= n
+    k0 = int((_unit_f32(seed, 1)[0] * np.float32(n - count)).astype(np.uint32))
+    k0 = max(1, k0)
+    k = np.arange(k0, k0 + count, dtype=np.uint64)
+    return np.clip(k, 1, n).astype(np.uint32)
 
 
 # ---------------------------------------------------------------- batches

@@ -102,12 +102,37 @@ int ggks_bucket_multiselect_i32(const int32_t *d_vec, int64_t n,
 int ggks_radix_multiselect(ggks_ctx *ctx, int dtype, const void *d_vec,
                            int64_t n, const uint32_t *h_k, int64_t num_k,
                            void *h_out, const ggks_opts *opt);
+/* Typed forms on the default context (the reference's per-type overloads,
+ * merrillSelect.cu:210-259 (uint, float), inplaceRadixSelect.cu:83-108 (uint,
+ * float, double)); i32 through the same key conversion. */
+int ggks_radix_multiselect_f32(const float *d_vec, int64_t n,
+                               const uint32_t *h_k, int32_t num_k,
+                               float *h_out, const ggks_opts *opt);
+int ggks_radix_multiselect_f64(const double *d_vec, int64_t n,
+                               const uint32_t *h_k, int32_t num_k,
+                               double *h_out, const ggks_opts *opt);
+int ggks_radix_multiselect_u32(const uint32_t *d_vec, int64_t n,
+                               const uint32_t *h_k, int32_t num_k,
+                               uint32_t *h_out, const ggks_opts *opt);
+int ggks_radix_multiselect_i32(const int32_t *d_vec, int64_t n,
+                               const uint32_t *h_k, int32_t num_k,
+                               int32_t *h_out, const ggks_opts *opt);
 
 /* Replaces randomizedTopkSelectWrapper (randomizedTopkSelect.cu:210-213) and
  * timeBucketTopkselect (multiselectTimingFunctions.cu:204-231): the K largest
  * values, descending, into the caller's host buffer h_out[K]. */
 int ggks_topk(ggks_ctx *ctx, int dtype, const void *d_vec, int64_t n,
               int64_t K, void *h_out, const ggks_opts *opt);
+/* Typed forms on the default context (randomizedTopkSelectWrapper<T>,
+ * randomizedTopkSelect.cu:210-213, for T = float, double, uint; plus int). */
+int ggks_topk_f32(const float *d_vec, int64_t n, int32_t K, float *h_out,
+                  const ggks_opts *opt);
+int ggks_topk_f64(const double *d_vec, int64_t n, int32_t K, double *h_out,
+                  const ggks_opts *opt);
+int ggks_topk_u32(const uint32_t *d_vec, int64_t n, int32_t K, uint32_t *h_out,
+                  const ggks_opts *opt);
+int ggks_topk_i32(const int32_t *d_vec, int64_t n, int32_t K, int32_t *h_out,
+                  const ggks_opts *opt);
 
 /* ---- stream-ordered variants: device k list and device outputs ------
  * Nothing is synchronised and no host memory is touched, so these can be
@@ -269,6 +294,8 @@ typedef struct ggks_server_stats {
   uint64_t scratch_budget;     /* bytes */
   uint64_t scratch_in_use;     /* bytes now */
   uint64_t peak_scratch_bytes; /* most bytes in use after an admission */
+  uint64_t failed;             /* requests of batches whose kernels reported an
+                                  internal error (latency -1, output untouched) */
 } ggks_server_stats;
 
 typedef struct ggks_server ggks_server;
@@ -283,13 +310,16 @@ int ggks_server_create(ggks_server **server, int device, const void *const *resi
 /* Queue a request (validated now; GGKS_ENOMEM if it alone exceeds the budget). */
 int ggks_server_submit(ggks_server *server, const ggks_request *req);
 /* Schedule, then report up to max_done completed requests (tag and latency in
- * seconds from submission); returns how many were written. */
+ * seconds from submission, or -1 for a request whose batch reported an
+ * internal error; its output is untouched); returns how many were written.
+ * Request slots are recycled once reported, so a long-running server's host
+ * memory stays bounded by the requests in flight. */
 int ggks_server_poll(ggks_server *server, uint64_t *tags, double *latency_s, int max_done);
 /* Run until nothing is pending or in flight. */
 int ggks_server_drain(ggks_server *server);
 /* Replay an open-loop arrival trace (arrival_s ascending, seconds from the
  * call): request i is submitted at arrival_s[i]; latency_s[i] = completion
- * seen - arrival_s[i], or -1 if the request was refused. */
+ * seen - arrival_s[i], or -1 if the request was refused or failed. */
 int ggks_server_run_trace(ggks_server *server, const ggks_request *reqs, const double *arrival_s,
                           int64_t n, double *latency_s);
 int ggks_server_stats_get(ggks_server *server, ggks_server_stats *stats);
@@ -298,15 +328,31 @@ int ggks_server_destroy(ggks_server *server);
 /* ---- seeded synthetic inputs on the device ------------------------------
  * Replaces the reference's cuRAND problem generators (generateProblems.cu:
  * 25-428, clock-seeded there).  Element i is drawn from output start + i of
- * the SplitMix64 stream of `seed`.  family 0 = uniform: f32 on the 2^-24
- * grid of [0,1), f64 on the 2^-53 grid, u32/i32 uniform in [lo, hi]
- * (bit-identical to the Python package's host generators); family 1 =
- * normal(0,1) by Box-Muller in double (f32/f64; the device's libm may differ
- * from the host's in the last bit); family 2 = levels: f32/f64 holding the
- * integers lo..hi (0 <= lo <= hi < 2^32), heavy duplicates; family 3 =
- * Cauchy(0,1), tan(pi (u - 1/2)) clipped to finite values (f32/f64);
- * family 4 = plateau: 0.5 with probability 1/2, else uniform [0,1)
- * (generateProblems.cu's adversarial shapes).  Synchronous. */
+ * the SplitMix64 stream of `seed`.  Families (integer dtypes take 0, 2, 5,
+ * 10 and 11; float dtypes all):
+ *   0 uniform: f32 on the 2^-24 grid of [0,1), f64 on the 2^-53 grid,
+ *     u32/i32 uniform in [lo, hi] (bit-identical to the Python package's host
+ *     generators)
+ *   1 normal(0,1) by Box-Muller in double (the device's libm may differ from
+ *     the host's in the last bit)
+ *   2 levels: floats holding the integers lo..hi (0 <= lo <= hi < 2^32),
+ *     heavy duplicates (integers: as family 0)
+ *   3 Cauchy(0,1), tan(pi (u - 1/2)) clipped to finite values
+ *   4 plateau: 0.5 with probability 1/2, else uniform [0,1)
+ *   5 sorted: position j (= start + i) of a non-decreasing sequence of
+ *     length hi, floor((j 2^32 + r_j) / hi) with r_j < 2^32 from the stream
+ *     (i32: minus 2^31; floats: times 2^-32); needs start + n <= hi
+ *     (generateProblems.cu:34-45)
+ *   6 bucket killer: masked uniform bits (f32 & 0x80EFFFFF, f64 &
+ *     0x800FFFFFFFFFFFFF) with positions 0..252 (f64: 0..2044) holding the
+ *     powers of two 2^-126..2^126 (2^-1022..2^1022) (:344-367, :385-423)
+ *   7 noisy: 1 with probability 1/5 else 0, plus N(0, 0.01) (:249-277)
+ *   8 half-normal |N(0,1)| (:322-341)
+ *   9 huge: uniform * 10^6 (f32 product in float) (:287-298)
+ *  10 ones: every value 1 (:180-197)
+ *  11 ones-twos: 1 or 2 with probability 1/2 (:154-177)
+ *  12 normal(0, 100^2) (:300-306)
+ * Synchronous. */
 int ggks_generate(void *d_ptr, int dtype, int family, uint64_t seed, uint64_t start, uint64_t n,
                   int64_t lo, int64_t hi);
 

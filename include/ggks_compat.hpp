@@ -44,17 +44,18 @@ namespace BucketMultiselect {
 // the first `length` elements of the device vector d_vector; kVals_ori and
 // outputs are host arrays of numKs entries.  blocks / threads are hints
 // (0 = tuned), as the reference takes them from the device properties.
-// Returns 1, the reference's return value.
+// Returns T(1), the reference's return value (its `return 1` from a function
+// returning T, bucketMultiselect.cu:837-838, :870).
 template <typename T>
-int bucketMultiselectWrapper(T* d_vector, int length, unsigned int* kVals_ori, int numKs,
-                             T* outputs, int blocks, int threads) {
+T bucketMultiselectWrapper(T* d_vector, int length, unsigned int* kVals_ori, int numKs,
+                           T* outputs, int blocks, int threads) {
   ggks_opts o;
   std::memset(&o, 0, sizeof o);
   o.blocks = blocks;
   o.threads = threads;
   ggks_compat::check(ggks_bucket_multiselect(nullptr, ggks_compat::dtype_of<T>::value, d_vector,
                                              length, kVals_ori, numKs, outputs, &o));
-  return 1;
+  return (T)1;
 }
 
 }  // namespace BucketMultiselect
@@ -73,6 +74,25 @@ T RadixSelectWrapper(T* d_vec, unsigned int size, unsigned int k) {
 }
 
 }  // namespace RadixSelect
+
+namespace InplaceRadix {
+
+// inplaceRadixSelect.cu:83-108: the k-th largest of the first `size` elements
+// of the device vector (uint, float and double overloads, as in the
+// reference; the reference converts the vector in place, this leaves it
+// intact).
+inline unsigned int inplaceRadixSelectWrapper(unsigned int* d_vec, unsigned int size,
+                                              unsigned int k) {
+  return RadixSelect::RadixSelectWrapper<unsigned int>(d_vec, size, k);
+}
+inline float inplaceRadixSelectWrapper(float* d_vec, unsigned int size, unsigned int k) {
+  return RadixSelect::RadixSelectWrapper<float>(d_vec, size, k);
+}
+inline double inplaceRadixSelectWrapper(double* d_vec, unsigned int size, unsigned int k) {
+  return RadixSelect::RadixSelectWrapper<double>(d_vec, size, k);
+}
+
+}  // namespace InplaceRadix
 
 // randomizedTopkSelect.cu:210-213: the k largest of the first `size`
 // elements of the device vector `list`, descending, in a malloc'd host array

@@ -25,6 +25,44 @@ static std::vector<unsigned int> percentiles(int n) {
   return k;
 }
 
+// typed C entry points of ggks.h per key type
+static int typed_radix(float* d, int64_t n, const uint32_t* k, int32_t nk, float* o) {
+  return ggks_radix_multiselect_f32(d, n, k, nk, o, nullptr);
+}
+static int typed_radix(double* d, int64_t n, const uint32_t* k, int32_t nk, double* o) {
+  return ggks_radix_multiselect_f64(d, n, k, nk, o, nullptr);
+}
+static int typed_radix(unsigned int* d, int64_t n, const uint32_t* k, int32_t nk, unsigned int* o) {
+  return ggks_radix_multiselect_u32(d, n, k, nk, o, nullptr);
+}
+static int typed_radix(int* d, int64_t n, const uint32_t* k, int32_t nk, int* o) {
+  return ggks_radix_multiselect_i32(d, n, k, nk, o, nullptr);
+}
+static int typed_topk(float* d, int64_t n, int32_t K, float* o) {
+  return ggks_topk_f32(d, n, K, o, nullptr);
+}
+static int typed_topk(double* d, int64_t n, int32_t K, double* o) {
+  return ggks_topk_f64(d, n, K, o, nullptr);
+}
+static int typed_topk(unsigned int* d, int64_t n, int32_t K, unsigned int* o) {
+  return ggks_topk_u32(d, n, K, o, nullptr);
+}
+static int typed_topk(int* d, int64_t n, int32_t K, int* o) {
+  return ggks_topk_i32(d, n, K, o, nullptr);
+}
+
+// InplaceRadix::inplaceRadixSelectWrapper has uint, float and double
+// overloads only (inplaceRadixSelect.cu:83-108)
+template <typename T>
+static int inplace_check(T* d, int n, unsigned int k, const std::vector<T>& h) {
+  const T v = InplaceRadix::inplaceRadixSelectWrapper(d, (unsigned)n, k);
+  return !same(v, h[(size_t)n - k]);
+}
+template <>
+int inplace_check<int>(int*, int, unsigned int, const std::vector<int>&) {
+  return 0;
+}
+
 template <typename T>
 static int problem(const char* name, int family, uint64_t seed, int n, int64_t lo, int64_t hi,
                    const std::vector<unsigned int>& ks, int K) {
@@ -37,13 +75,23 @@ static int problem(const char* name, int family, uint64_t seed, int n, int64_t l
   int bad = 0;
   std::vector<unsigned int> k = ks;
   std::vector<T> out(k.size());
-  BucketMultiselect::bucketMultiselectWrapper<T>(d, n, k.data(), (int)k.size(), out.data(), 0, 0);
+  // the reference's wrapper returns T(1) (bucketMultiselect.cu:837-838, :870)
+  const T one = BucketMultiselect::bucketMultiselectWrapper<T>(d, n, k.data(), (int)k.size(),
+                                                              out.data(), 0, 0);
+  bad += !(one == (T)1);
   for (size_t i = 0; i < k.size(); ++i) bad += !same(out[i], h[(size_t)n - k[i]]);
   const T r = RadixSelect::RadixSelectWrapper<T>(d, (unsigned)n, k[k.size() / 2]);
   bad += !same(r, h[(size_t)n - k[k.size() / 2]]);
   T* top = randomizedTopkSelectWrapper<T>(d, n, K);
   for (int j = 0; j < K; ++j) bad += !same(top[j], h[(size_t)n - 1 - j]);
   std::free(top);
+  // the typed C entry points (radix multiselect over every k, top-K)
+  std::vector<T> rout(k.size()), tout((size_t)K);
+  ggks_compat::check(typed_radix(d, n, k.data(), (int32_t)k.size(), rout.data()));
+  for (size_t i = 0; i < k.size(); ++i) bad += !same(rout[i], h[(size_t)n - k[i]]);
+  ggks_compat::check(typed_topk(d, n, K, tout.data()));
+  for (int j = 0; j < K; ++j) bad += !same(tout[j], h[(size_t)n - 1 - j]);
+  bad += inplace_check<T>(d, n, k[0], h);
   // the input must be intact (the reference's radix paths overwrite it)
   std::vector<T> again((size_t)n);
   ggks_compat::check(ggks_d2h(again.data(), d, sizeof(T) * (size_t)n));

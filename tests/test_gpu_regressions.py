@@ -1,0 +1,199 @@
+"""GPU regression tests for fixed defects and for the generator families.
+
+- radix digit widths 4..7 (bucket budgets below 2^8 once left the bucket
+  tables partly unloaded): full-range u32 / i32 keys with k in {1, 2, n/2,
+  n}, single vector and batched;
+- a batch whose engine segments hold more than 2^32 keys in total (the
+  compaction once kept 32-bit positions in the reduced buffer): overlapping
+  windows of one "sorted" vector, whose order statistics are single elements
+  of the sequence and are computed alone on the host (no 5e9-key sort);
+- process exit with live contexts, batches and servers (no destroy calls);
+- every generator family: device output against the host mirror where the
+  two are bit-identical, and bucket / radix multiselect against the oracle
+  on the read-back data with the reference's k families;
+- the Python server's tags: repeated caller tags each come back once.
+"""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, bits
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+@pytest.mark.parametrize("rb", [4, 5, 6, 7])
+def test_small_radix_digits(pkg, O, rb):
+    W = pkg.workloads
+    n = (1 << 20) + 7
+    for dtype, x in ((np.uint32, W.uint_range(60 + rb, n, 0, 2**32 - 1)),
+                     (np.int32, W.int_range(61 + rb, n, -2**31, 2**31 - 1))):
+        _say(f"radix_bits={rb} {np.dtype(dtype).name} n={n}")
+        # the extreme keys land in the first and last digit buckets
+        x[3] = np.iinfo(dtype).max
+        x[11] = np.iinfo(dtype).min
+        k = np.array([1, 2, n // 2, n - 1, n], np.uint32)
+        d = pkg.DeviceArray.from_host(x)
+        for kind in ("radix", "bucket"):
+            got = pkg.multiselect(d, k, kind=kind, radix_bits=rb)
+            assert np.array_equal(bits(got), bits(O.sort_choose(x, k))), (kind, rb)
+        # batched: three radix segments of different lengths back to back
+        lens = [n, 70000, 300001]
+        xs = [x] + [W.uint_range(62 + i, m, 0, 2**32 - 1).astype(np.uint32).view(dtype)
+                    for i, m in enumerate(lens[1:])]
+        xb = np.concatenate(xs)
+        ks = [np.array([1, 2, m // 2, m], np.uint32) for m in lens]
+        table = pkg.segment_table(lens, ["radix"] * 3, [4, 4, 4])
+        b = pkg.Batch(dtype, table, np.concatenate(ks), radix_bits=rb)
+        got = b(pkg.DeviceArray.from_host(xb))
+        want = np.concatenate([O.sort_choose(xi, ki) for xi, ki in zip(xs, ks)])
+        assert np.array_equal(bits(got), bits(want)), ("batch", rb)
+        b.close()
+
+
+def test_batch_beyond_2p32_keys(pkg):
+    """Five engine segments of 2^30 keys each (5.4e9 in total) over one
+    sorted u32 vector: segment i is the window [i 2^27, i 2^27 + 2^30), whose
+    k-th largest is element i 2^27 + 2^30 - k of the sequence."""
+    W = pkg.workloads
+    L, step, nseg, seed = 1 << 30, 1 << 27, 5, 4242
+    total = L + (nseg - 1) * step
+    _say(f"generating sorted u32 of {total} keys")
+    d = pkg.generate(np.uint32, "sorted", seed, total, hi=total)
+    kinds = ["bucket", "topk", "radix", "bucket", "topk"]
+    ks, nums = [], []
+    for i, kd in enumerate(kinds):
+        if kd == "topk":
+            nums.append(300)
+        else:
+            kk = np.concatenate([[1, 2, L // 2, L - 1, L],
+                                 W.k_uniform(900 + i, L, 59, dedup=False)]).astype(np.uint32)
+            ks.append(kk)
+            nums.append(kk.size)
+    table = pkg.segment_table([L] * nseg, kinds, nums,
+                              offsets=np.arange(nseg, dtype=np.uint64) * np.uint64(step))
+    _say(f"batch of {nseg} x 2^30 keys, {int(table['num_k'].sum())} outputs")
+    b = pkg.Batch(np.uint32, table, np.concatenate(ks))
+    st = b.stats()
+    assert st["engine_segments"] == nseg and st["elements"] == nseg * L
+    got = b(d)
+    j = 0
+    for i, kd in enumerate(kinds):
+        base = i * step
+        if kd == "topk":
+            pos = base + L - 1 - np.arange(300)
+        else:
+            pos = base + L - ks[j].astype(np.int64)
+            j += 1
+        want = np.array([W.sorted_u32(seed, 1, int(p), total)[0] for p in pos], np.uint32)
+        o = int(table["out_offset"][i])
+        g = got[o:o + int(table["num_k"][i])]
+        bad = np.nonzero(g != want)[0]
+        assert bad.size == 0, (i, kd, bad[:5], g[bad[:5]], want[bad[:5]])
+    b.close()
+    d.free()
+
+
+def test_exit_with_live_objects():
+    """A process that leaves contexts, a batch, a server and device arrays
+    alive at exit (and one that destroys them) ends with status 0."""
+    code = r"""
+import sys, numpy as np
+sys.path.insert(0, %r)
+from __graft_entry__ import load_package
+pkg = load_package()
+W = pkg.workloads
+x = W.uniform_f32(3, 1 << 20)
+d = pkg.DeviceArray.from_host(x)
+ctx = pkg.Context(0)
+dk = pkg.DeviceArray.from_host(np.array([1, 5, 99], np.uint32))
+dout = pkg.DeviceArray(3, np.float32)
+pkg.multiselect_async(d, dk, dout, ctx)
+pkg.multiselect(d, [1, 2, 3])          # the default context
+t = pkg.segment_table([1 << 19, 1 << 19], ["bucket", "topk"], [2, 8])
+b = pkg.Batch(np.float32, t, np.array([1, 7], np.uint32))
+o = pkg.DeviceArray(10, np.float32)
+b.run(d, o, graph=True)
+srv = pkg.Server({np.float32: d}, scratch_budget=64 << 20)
+out = np.zeros(4, np.float32)
+srv.submit(np.float32, 0, 1 << 20, out, k=[1, 2, 3, 4])
+srv.drain()
+if sys.argv[1] == "destroy":
+    srv.close(); b.close(); ctx.close(); d.free()
+print("exit-ok", flush=True)
+""" % ROOT
+    for mode in ("leave", "destroy"):
+        _say(f"subprocess exit test ({mode})")
+        r = subprocess.run([sys.executable, "-c", code, mode], capture_output=True, text=True,
+                           timeout=300)
+        assert r.returncode == 0 and "exit-ok" in r.stdout, (mode, r.returncode, r.stderr[-2000:])
+
+
+FAMILIES_F = ["sorted", "killer", "noisy", "half_normal", "huge", "ones", "ones_twos",
+              "normal100"]
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64, np.uint32, np.int32])
+def test_generator_families_and_k_families(pkg, O, dtype):
+    W = pkg.workloads
+    n = (1 << 20) + 3
+    fams = FAMILIES_F if dtype in (np.float32, np.float64) else ["sorted", "ones", "ones_twos"]
+    kfam = {"even": W.k_even(1024, n), "normal": W.k_normal(7, 1024, n),
+            "cluster9": W.k_cluster9(8, 1024, n), "sectioned": W.k_sectioned(9, 1024, n),
+            "uniform": W.k_uniform(10, n, 1024, dedup=False)}
+    for fam in fams:
+        _say(f"family {fam} {np.dtype(dtype).name}")
+        d = pkg.generate(dtype, fam, 77, n, hi=n if fam == "sorted" else 0)
+        x = d.to_host()
+        # host mirrors that are bit-identical
+        if fam == "sorted":
+            assert np.array_equal(bits(x), bits(W.sorted_as(dtype, W.sorted_u32(77, n))))
+            assert np.all(np.diff(x.astype(np.float64)) >= 0)
+        elif fam == "killer":
+            h = W.killer_f32(77, n) if dtype == np.float32 else W.killer_f64(77, n)
+            assert np.array_equal(bits(x), bits(h))
+        elif fam == "huge" and dtype == np.float32:
+            assert np.array_equal(bits(x), bits(W.huge_f32(77, n)))
+        elif fam == "ones":
+            assert np.all(x == 1)
+        elif fam == "ones_twos":
+            assert np.array_equal(bits(x), bits(W.ones_twos(77, n, dtype)))
+        elif fam == "half_normal":
+            assert np.all(x >= 0)
+        for kname, k in kfam.items():
+            for kind in ("bucket", "radix"):
+                got = pkg.multiselect(d, k, kind=kind)
+                want = O.sort_choose(x, k)
+                bad = np.nonzero(bits(got) != bits(want))[0]
+                assert bad.size == 0, (fam, kname, kind, bad[:3])
+        t = pkg.topk(d, 777)
+        assert np.array_equal(bits(t), bits(O.topk(x, 777))), (fam, "topk")
+        d.free()
+
+
+def test_server_repeated_tags(pkg):
+    W = pkg.workloads
+    x = W.uniform_f32(5, 1 << 18)
+    d = pkg.DeviceArray.from_host(x)
+    srt = np.sort(x)
+    srv = pkg.Server({np.float32: d}, scratch_budget=256 << 20, max_inflight=2, max_batch=8)
+    outs = []
+    for i in range(40):
+        out = np.zeros(3, np.float32)
+        k = np.array([1 + i, 100 + i, 1000 + i], np.uint32)
+        srv.submit(np.float32, 0, x.size, out, k=k, tag=7)  # the same caller tag every time
+        outs.append((k, out))
+    srv.drain()
+    done = srv.poll()
+    assert len(done) == 40 and all(t == 7 and lat >= 0 for t, lat in done)
+    for k, out in outs:
+        assert np.array_equal(out, srt[x.size - k.astype(np.int64)])
+    assert srv.stats()["failed"] == 0
+    srv.close()
