@@ -29,6 +29,7 @@ EXPORTS = [
     "ggks_topk_i32",
     "ggks_bucket_multiselect_async", "ggks_radix_multiselect_async", "ggks_topk_async",
     "ggks_ctx_reserve", "ggks_ctx_check", "ggks_ctx_stream", "ggks_dev_alloc",
+    "ggks_bucket_multiselect_noalloc",
     "ggks_dev_free", "ggks_h2d", "ggks_d2h", "ggks_d2d", "ggks_sync",
     "ggks_device_count", "ggks_set_device", "ggks_dev_fill", "ggks_ctx_profile_arm",
     "ggks_ctx_profile_read", "ggks_ctx_profile_marks", "ggks_batch_create", "ggks_batch_scratch_bytes", "ggks_batch_run",
@@ -45,6 +46,7 @@ GEN_FAMILIES = {"uniform": 0, "normal": 1, "levels": 2, "cauchy": 3, "plateau": 
                 "killer": 6, "noisy": 7, "half_normal": 8, "huge": 9, "ones": 10,
                 "ones_twos": 11, "normal100": 12}
 NSTAGES = 8
+NOALLOC_SCRATCH_BYTES = 64 << 20  # GGKS_NOALLOC_SCRATCH_BYTES
 STAGES = ["queries", "sample", "lut", "count", "reduce", "findk", "compact", "finish"]
 
 
@@ -132,6 +134,7 @@ def lib() -> ctypes.CDLL:
         "ggks_radix_multiselect_async": ([vp, i32, vp, i64, vp, i64, vp, optp], i32),
         "ggks_topk_async": ([vp, i32, vp, i64, i64, vp, optp], i32),
         "ggks_ctx_reserve": ([vp, i32, i32, i64, i64], i32),
+        "ggks_bucket_multiselect_noalloc": ([i32, vp, i64, vp, i64, vp, optp, vp], i32),
         "ggks_ctx_check": ([vp], i32),
         "ggks_ctx_stream": ([vp], vp),
         "ggks_dev_alloc": ([ctypes.POINTER(vp), sz], i32),

@@ -49,6 +49,11 @@ struct Plan {
   uint32_t nq = 0;
   uint32_t bmax_max = 0;
   uint32_t pbits_max = 0;
+  // table sizes (equal to the vectors' sizes when the tables are built)
+  uint32_t nsegs = 0, nchunks = 0, nitems = 0, nsblocks = 0, ndirect = 0;
+  // longest direct child, and the largest mean bucket size of an engine
+  // segment (a radix segment's whole length): picks the finisher variant
+  uint64_t dmax_keys = 0, emax_keys = 0;
   uint64_t seed = 0;
   // arena offsets (bytes)
   size_t off_tables = 0, off_segs = 0, off_chunks = 0, off_items = 0, off_sblocks = 0, tables_end = 0;
@@ -82,13 +87,103 @@ struct Plan {
   }
 };
 
-// Plan a single-vector select of nq ranks over n keys.  The engine's single
-// segment reads its queries straight from the k list (or as ranks of a
-// top-K); the small-n direct path takes explicit Query records.
-inline void plan_single(Plan& p, int kind, int dtype, uint64_t n, uint32_t nq,
-                        const ggks_opts& o) {
+// The shape of a single-vector plan: its route, bucket function parameters,
+// chunking and sample size, computed from scalars alone (no tables).
+struct SingleShape {
+  bool direct;        // the whole vector fits the finisher's LDS
+  uint32_t mode, rb;  // SegMode and radix digit width
+  uint32_t bmax, pb;  // bucket budget, LUT prefix bits
+  uint32_t ns;        // samples
+  uint32_t nitems;    // reduce items
+  uint32_t gl;        // log2 groups per reduce item
+  uint64_t G, csz;    // chunks and chunk length
+};
+
+inline SingleShape single_shape(int kind, int dtype, uint64_t n, const ggks_opts& o) {
+  SingleShape sh = {};
   const int eb = dtype_bytes(dtype);
   const uint64_t cap = kFinishLdsBytes / (uint32_t)eb;
+  if (n <= cap) {
+    sh.direct = true;
+    return sh;
+  }
+  Tuning t = default_tuning(n, eb);
+  uint32_t bl;
+  sh.mode = MODE_SAMPLE;
+  if (kind == GGKS_KIND_RADIX) {
+    sh.mode = MODE_RADIX;
+    uint32_t rb = o.radix_bits ? (uint32_t)o.radix_bits : 14u;
+    sh.rb = std::min<uint32_t>(14, std::max<uint32_t>(4, rb));
+    bl = radix_budget_log2(sh.rb);
+    sh.pb = std::min<uint32_t>(kMaxPrefixBits, sh.rb);
+  } else {
+    bl = o.buckets_log2 ? (uint32_t)o.buckets_log2 : t.buckets_log2;
+    bl = std::min<uint32_t>(15, std::max<uint32_t>(8, bl));
+    sh.pb = std::min<uint32_t>(kMaxPrefixBits, bl - 2);
+  }
+  sh.bmax = 1u << bl;
+  uint64_t G = o.blocks > 0 ? (uint64_t)o.blocks : t.chunks;
+  G = std::max<uint64_t>(1, std::min<uint64_t>(G, (n + 4095) / 4096));
+  G = std::min<uint64_t>(G, kMaxChunks);
+  uint64_t csz = (n + G - 1) / G;
+  csz = (csz + 63) / 64 * 64;
+  sh.G = (n + csz - 1) / csz;
+  sh.csz = csz;
+  if (sh.mode == MODE_SAMPLE) {
+    uint64_t want = (n / 256 + 1023) / 1024 * 1024;
+    sh.ns = (uint32_t)std::min<uint64_t>(65536, std::max<uint64_t>(4096, want));
+  }
+  sh.gl = reduce_groups_log2(sh.G);
+  const uint32_t ng = (sh.bmax + kReduceBins - 1) / kReduceBins;
+  sh.nitems = (ng + (1u << sh.gl) - 1) >> sh.gl;
+  return sh;
+}
+
+// The descriptor of a single-vector plan's one engine segment.
+inline SegDesc single_segment(const SingleShape& sh, int kind, uint64_t n, uint32_t nq) {
+  SegDesc s = {};
+  s.data_off = 0;
+  s.len = n;
+  s.row_base = 0;
+  s.red_base = 0;
+  s.bmax = sh.bmax;
+  s.pbits = sh.pb;
+  s.nchunks = (uint32_t)sh.G;
+  s.sample_off = 0;
+  s.nsample = sh.ns;
+  s.lut_off = 0;
+  s.tot_off = 0;
+  s.bits_off = 0;
+  s.q_off = 0;
+  s.q_cnt = nq;
+  s.mode = sh.mode;
+  s.radix_bits = sh.rb;
+  s.qsrc = kind == GGKS_KIND_TOPK ? QSRC_TOPK : QSRC_K;
+  s.out_off = 0;
+  return s;
+}
+
+// The direct plan's one child: the whole vector, range found by the finisher.
+inline Child single_direct_child(uint64_t n, uint32_t nq) {
+  Child c = {};
+  c.start = 0;
+  c.len = n;
+  c.q_off = 0;
+  c.q_cnt = nq;
+  c.src = 0;
+  c.need_range = 1;
+  return c;
+}
+
+// Plan a single-vector select of nq ranks over n keys.  The engine's single
+// segment reads its queries straight from the k list (or as ranks of a
+// top-K); the small-n direct path takes explicit Query records.  With
+// tables = false only the arena layout is computed (the table vectors stay
+// empty and allocate nothing): the caller materialises the tables itself.
+inline void plan_single(Plan& p, int kind, int dtype, uint64_t n, uint32_t nq,
+                        const ggks_opts& o, bool tables = true) {
+  const int eb = dtype_bytes(dtype);
+  const SingleShape sh = single_shape(kind, dtype, n, o);
   p.nq = nq;
   p.seed = o.seed ? o.seed : 0x6A09E667F3BCC909ull;
   size_t off = 0;
@@ -97,16 +192,11 @@ inline void plan_single(Plan& p, int kind, int dtype, uint64_t n, uint32_t nq,
     off = align_up(off + bytes);
     return at;
   };
-  if (n <= cap) {
+  if (sh.direct) {
     p.direct = true;
-    Child c = {};
-    c.start = 0;
-    c.len = n;
-    c.q_off = 0;
-    c.q_cnt = nq;
-    c.src = 0;
-    c.need_range = 1;
-    p.direct_children.push_back(c);
+    p.ndirect = 1;
+    p.dmax_keys = n;
+    if (tables) p.direct_children.push_back(single_direct_child(n, nq));
     p.off_tables = off;
     p.off_dchildren = take(sizeof(Child));
     p.tables_end = off;
@@ -114,74 +204,37 @@ inline void plan_single(Plan& p, int kind, int dtype, uint64_t n, uint32_t nq,
     p.total = off;
     return;
   }
-  Tuning t = default_tuning(n, eb);
-  uint32_t mode = MODE_SAMPLE, rb = 0, bl, pb;
-  if (kind == GGKS_KIND_RADIX) {
-    mode = MODE_RADIX;
-    rb = o.radix_bits ? (uint32_t)o.radix_bits : 14u;
-    rb = std::min<uint32_t>(14, std::max<uint32_t>(4, rb));
-    bl = radix_budget_log2(rb);
-    pb = std::min<uint32_t>(kMaxPrefixBits, rb);
-  } else {
-    bl = o.buckets_log2 ? (uint32_t)o.buckets_log2 : t.buckets_log2;
-    bl = std::min<uint32_t>(15, std::max<uint32_t>(8, bl));
-    pb = std::min<uint32_t>(kMaxPrefixBits, bl - 2);
-  }
-  const uint32_t bmax = 1u << bl;
-  uint64_t G = o.blocks > 0 ? (uint64_t)o.blocks : t.chunks;
-  G = std::max<uint64_t>(1, std::min<uint64_t>(G, (n + 4095) / 4096));
-  G = std::min<uint64_t>(G, kMaxChunks);
-  uint64_t csz = (n + G - 1) / G;
-  csz = (csz + 63) / 64 * 64;
-  G = (n + csz - 1) / csz;
-  uint32_t ns = 0;
-  if (mode == MODE_SAMPLE) {
-    uint64_t want = (n / 256 + 1023) / 1024 * 1024;
-    ns = (uint32_t)std::min<uint64_t>(65536, std::max<uint64_t>(4096, want));
-  }
-  SegDesc s = {};
-  s.data_off = 0;
-  s.len = n;
-  s.row_base = 0;
-  s.red_base = 0;
-  s.bmax = bmax;
-  s.pbits = pb;
-  s.nchunks = (uint32_t)G;
-  s.sample_off = 0;
-  s.nsample = ns;
-  s.lut_off = 0;
-  s.tot_off = 0;
-  s.bits_off = 0;
-  s.q_off = 0;
-  s.q_cnt = nq;
-  s.mode = mode;
-  s.radix_bits = rb;
-  s.qsrc = kind == GGKS_KIND_TOPK ? QSRC_TOPK : QSRC_K;
-  s.out_off = 0;
-  p.segs.push_back(s);
-  for (uint64_t i = 0; i < G; ++i) {
-    Chunk c = {};
-    c.seg = 0;
-    c.idx = (uint32_t)i;
-    c.start = i * csz;
-    c.len = std::min<uint64_t>(csz, n - i * csz);
-    p.chunks.push_back(c);
-  }
-  {
-    const uint32_t gl = reduce_groups_log2(G), ng = (bmax + kReduceBins - 1) / kReduceBins;
-    for (uint32_t g = 0; g < ng; g += 1u << gl) p.items.push_back({0u, (uint16_t)g, (uint16_t)gl});
+  const uint32_t bmax = sh.bmax, pb = sh.pb, ns = sh.ns;
+  const uint64_t G = sh.G;
+  if (tables) {
+    p.segs.push_back(single_segment(sh, kind, n, nq));
+    for (uint64_t i = 0; i < G; ++i) {
+      Chunk c = {};
+      c.seg = 0;
+      c.idx = (uint32_t)i;
+      c.start = i * sh.csz;
+      c.len = std::min<uint64_t>(sh.csz, n - i * sh.csz);
+      p.chunks.push_back(c);
+    }
+    for (uint32_t i = 0; i < sh.nitems; ++i)
+      p.items.push_back({0u, (uint16_t)(i << sh.gl), (uint16_t)sh.gl});
+    for (uint32_t y = 0; y < ns / kThreads; ++y) p.sblocks.push_back({0u, y});
   }
   p.bmax_max = bmax;
   p.pbits_max = pb;
-  for (uint32_t y = 0; y < ns / kThreads; ++y) p.sblocks.push_back({0u, y});
+  p.nsegs = 1;
+  p.nchunks = (uint32_t)G;
+  p.nitems = sh.nitems;
+  p.nsblocks = ns / kThreads;
+  p.emax_keys = sh.mode == MODE_RADIX ? n : (n + bmax - 1) / bmax;
 
   p.off_tables = off;
-  p.off_segs = take(p.segs.size() * sizeof(SegDesc));
-  p.off_chunks = take(p.chunks.size() * sizeof(Chunk));
-  p.off_items = take(p.items.size() * sizeof(ReduceItem));
-  p.off_sblocks = take(p.sblocks.size() * sizeof(SampleBlock));
+  p.off_segs = take(sizeof(SegDesc));
+  p.off_chunks = take((size_t)G * sizeof(Chunk));
+  p.off_items = take((size_t)sh.nitems * sizeof(ReduceItem));
+  p.off_sblocks = take((size_t)(ns / kThreads) * sizeof(SampleBlock));
   p.tables_end = off;
-  p.off_states = take(p.segs.size() * sizeof(SegState));
+  p.off_states = take(sizeof(SegState));
   p.off_samples = take((size_t)std::max<uint32_t>(ns, 1) * eb);
   p.off_minmax = take((size_t)std::max<uint32_t>(ns / 1024, 1) * 2 * eb);
   p.off_lut = take((size_t)(1u << pb) * 4);
@@ -388,6 +441,15 @@ inline bool plan_batch(Plan& p, int dtype, const std::vector<BatchSeg>& in, cons
     bitc += (bmax + 31) / 32;
   }
   p.nq = (uint32_t)qoff;
+  p.nsegs = (uint32_t)p.segs.size();
+  p.nchunks = (uint32_t)p.chunks.size();
+  p.nitems = (uint32_t)p.items.size();
+  p.nsblocks = (uint32_t)p.sblocks.size();
+  p.ndirect = (uint32_t)p.direct_children.size();
+  for (const Child& c : p.direct_children) p.dmax_keys = std::max<uint64_t>(p.dmax_keys, c.len);
+  for (const SegDesc& sd : p.segs)
+    p.emax_keys = std::max<uint64_t>(
+        p.emax_keys, sd.mode == MODE_RADIX ? sd.len : (sd.len + sd.bmax - 1) / sd.bmax);
   size_t off = 0;
   auto take = [&](size_t bytes) {
     size_t at = off;

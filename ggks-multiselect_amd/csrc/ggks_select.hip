@@ -221,24 +221,20 @@ static int enqueue_level(const Exec& x, const Plan& p, const KT* d_src, KT* d_ou
                          const uint32_t* d_k) {
   char* A = x.arena;
   hipStream_t s = x.stream;
-  const uint32_t ndirect = (uint32_t)p.direct_children.size();
-  const bool beside = ndirect > 0 && !p.segs.empty() && x.side;
+  const uint32_t ndirect = p.ndirect;
+  const bool beside = ndirect > 0 && p.nsegs > 0 && x.side;
   // the register-light 256-thread finisher when every child a plan can
   // make is expected to fit its registers twice over (a longer child is
   // still finished exactly, from memory): direct children by their length,
   // an engine segment's by its mean bucket size
   const uint64_t lite_keys = finish_small_reg_keys<KT>(true);
-  uint64_t dmax = 0, emax = 0;
-  for (const Child& c : p.direct_children) dmax = std::max<uint64_t>(dmax, c.len);
-  for (const SegDesc& sd : p.segs)
-    emax = std::max<uint64_t>(emax, sd.mode == MODE_RADIX ? sd.len : (sd.len + sd.bmax - 1) / sd.bmax);
-  const bool dlite = dmax <= lite_keys, elite = 2 * emax <= lite_keys;
+  const bool dlite = p.dmax_keys <= lite_keys, elite = 2 * p.emax_keys <= lite_keys;
   auto finish_direct = [&](hipStream_t ds) {
     const Child* dk = reinterpret_cast<const Child*>(A + p.off_dchildren);
     const Query* dq = reinterpret_cast<const Query*>(A + p.off_dqueries);
     finish<KT, C>(ds, d_src, (const KT*)nullptr, dk, ndirect, dq, d_out, 256, dlite);
   };
-  const uint32_t nseg = (uint32_t)p.segs.size();
+  const uint32_t nseg = p.nsegs;
   if (nseg > 0) {
     const SegDesc* segs = reinterpret_cast<const SegDesc*>(A + p.off_segs);
     SegState* states = reinterpret_cast<SegState*>(A + p.off_states);
@@ -258,12 +254,12 @@ static int enqueue_level(const Exec& x, const Plan& p, const KT* d_src, KT* d_ou
     Child* kids = reinterpret_cast<Child*>(A + p.off_children);
     Query* cq = reinterpret_cast<Query*>(A + p.off_cq);
     KT* red = reinterpret_cast<KT*>(A + p.off_red);
-    const uint32_t nchunks = (uint32_t)p.chunks.size();
+    const uint32_t nchunks = p.nchunks;
 
     stage(x, 1);
-    if (!p.sblocks.empty()) {
+    if (p.nsblocks > 0) {
       const SampleBlock* sblocks = reinterpret_cast<const SampleBlock*>(A + p.off_sblocks);
-      hipLaunchKernelGGL((k_sample<KT, C>), dim3((uint32_t)p.sblocks.size()), dim3(kSampleThreads), 0, s,
+      hipLaunchKernelGGL((k_sample<KT, C>), dim3(p.nsblocks), dim3(kSampleThreads), 0, s,
                          d_src, segs, sblocks, samples, minmax, p.seed);
     }
     stage(x, 2);
@@ -280,7 +276,7 @@ static int enqueue_level(const Exec& x, const Plan& p, const KT* d_src, KT* d_ou
                        ((size_t)p.bmax_max + (1u << p.pbits_max)) * 4, s, d_src, segs, states,
                        chunks, lut, rows);
     stage(x, 4);
-    hipLaunchKernelGGL(k_reduce, dim3((uint32_t)p.items.size()), dim3(kThreads), 0, s, segs,
+    hipLaunchKernelGGL(k_reduce, dim3(p.nitems), dim3(kThreads), 0, s, segs,
                        states, items, rows, tot, gsum);
     stage(x, 5);
     if (p.bmax_max <= kFindkSmallBuckets)
@@ -458,6 +454,134 @@ static int select_one(ggks_ctx* c, int kind, int dtype, const void* d_vec, int64
 
 #include "ggks_batch.hpp"
 #include "ggks_serve.hpp"
+
+// ------------------------------------------------- no-allocation entry point
+// ggks_bucket_multiselect_noalloc runs the single-vector engine over a fixed
+// device array instead of a context arena: no hipMalloc, hipFree, pinned
+// buffer or host staging on any call.  Every GPU the library runs on holds
+// its own copy of `scratch` (kNoAllocScratchBytes) until the process ends;
+// host code reaches it only through the address hipGetSymbolAddress returns
+// for the current device (the array's name is a host-side stand-in).  All
+// calls on one GPU share that array, so calls that overlap on one GPU race.
+constexpr size_t kNoAllocScratchBytes = GGKS_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+static std::mutex g_noalloc_mu;
+static void* g_noalloc_ptr[64];
+
+// The device address of `scratch` on the current device, looked up once per
+// device with that device current.
+static int noalloc_base(char** out) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return fail(GGKS_EINVAL, "device %d out of range", dev);
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (!g_noalloc_ptr[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(GGKS_EDEVICE, "hipGetSymbolAddress(scratch) failed on device %d: %s", dev,
+                  hipGetErrorString(e));
+    g_noalloc_ptr[dev] = ptr;
+  }
+  *out = static_cast<char*>(g_noalloc_ptr[dev]);
+  return GGKS_OK;
+}
+
+// Everything the plan kernel writes, by value (byte offsets into the scratch).
+struct SinglePlanArgs {
+  SegDesc seg;
+  Child dchild;
+  uint64_t n, csz;
+  uint32_t G, nitems, gl, nsb, direct;
+  uint64_t off_segs, off_chunks, off_items, off_sblocks, off_dchildren, off_status;
+};
+
+// Materialise a single-vector plan's tables in device memory (what the
+// context path uploads from pinned memory) and clear the status block.
+// Thread i writes chunk i, reduce item i and sample block i.
+__global__ __launch_bounds__(256) void k_plan_single(char* __restrict__ base, SinglePlanArgs a) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 64) reinterpret_cast<uint32_t*>(base + a.off_status)[i] = 0u;
+  if (a.direct) {
+    if (i == 0) *reinterpret_cast<Child*>(base + a.off_dchildren) = a.dchild;
+    return;
+  }
+  if (i == 0) *reinterpret_cast<SegDesc*>(base + a.off_segs) = a.seg;
+  if (i < a.G) {
+    Chunk c;
+    c.seg = 0;
+    c.idx = i;
+    c.start = (uint64_t)i * a.csz;
+    c.len = min(a.csz, a.n - c.start);
+    reinterpret_cast<Chunk*>(base + a.off_chunks)[i] = c;
+  }
+  if (i < a.nitems) {
+    ReduceItem it;
+    it.seg = 0;
+    it.group = (uint16_t)(i << a.gl);
+    it.gi_log2 = (uint16_t)a.gl;
+    reinterpret_cast<ReduceItem*>(base + a.off_items)[i] = it;
+  }
+  if (i < a.nsb) {
+    SampleBlock sb;
+    sb.seg = 0;
+    sb.blk = i;
+    reinterpret_cast<SampleBlock*>(base + a.off_sblocks)[i] = sb;
+  }
+}
+
+static int select_noalloc(int dtype, const void* d_vec, int64_t n, const uint32_t* d_k,
+                          int64_t num_k, void* d_out, const ggks_opts* opt, hipStream_t stream) {
+  int rc = check_common(dtype, d_vec, n, num_k);
+  if (rc) return rc;
+  if (num_k > 0 && (!d_k || !d_out)) return fail(GGKS_EINVAL, "null k or output array");
+  if (num_k == 0) return GGKS_OK;
+  ggks_opts o = {};
+  if (opt) o = *opt;
+  Plan p;  // layout only: its table vectors stay empty
+  plan_single(p, GGKS_KIND_BUCKET, dtype, (uint64_t)n, (uint32_t)num_k, o, false);
+  const size_t off_status = align_up(p.total);
+  const size_t need = off_status + 256;
+  if (need > kNoAllocScratchBytes)
+    return fail(GGKS_ENOMEM,
+                "n=%lld %s with %lld k needs %zu bytes of scratch; the no-allocation entry "
+                "point has %zu (use the context entry points for larger selects)",
+                (long long)n, dtype == GGKS_F64 ? "f64" : "4-byte keys", (long long)num_k, need,
+                kNoAllocScratchBytes);
+  char* base = nullptr;
+  if ((rc = noalloc_base(&base))) return rc;
+  const SingleShape sh = single_shape(GGKS_KIND_BUCKET, dtype, (uint64_t)n, o);
+  SinglePlanArgs a = {};
+  a.n = (uint64_t)n;
+  a.direct = p.direct ? 1u : 0u;
+  if (p.direct) {
+    a.dchild = single_direct_child((uint64_t)n, (uint32_t)num_k);
+  } else {
+    a.seg = single_segment(sh, GGKS_KIND_BUCKET, (uint64_t)n, (uint32_t)num_k);
+    a.csz = sh.csz;
+    a.G = (uint32_t)sh.G;
+    a.nitems = sh.nitems;
+    a.gl = sh.gl;
+    a.nsb = sh.ns / kThreads;
+  }
+  a.off_segs = p.off_segs;
+  a.off_chunks = p.off_chunks;
+  a.off_items = p.off_items;
+  a.off_sblocks = p.off_sblocks;
+  a.off_dchildren = p.off_dchildren;
+  a.off_status = off_status;
+  const uint32_t nthreads = std::max<uint32_t>(64, std::max(a.G, std::max(a.nitems, a.nsb)));
+  hipLaunchKernelGGL(k_plan_single, dim3((nthreads + 255) / 256), dim3(256), 0, stream, base, a);
+  uint32_t* status = reinterpret_cast<uint32_t*>(base + off_status);
+  if (p.direct) {
+    const uint32_t nk = (uint32_t)num_k;
+    hipLaunchKernelGGL(k_queries, dim3((nk + 255) / 256), dim3(256), 0, stream, d_k, nk,
+                       (uint64_t)n, reinterpret_cast<Query*>(base + p.off_dqueries), status);
+  }
+  const Exec x = {base, stream, status, nullptr};
+  return launch(x, dtype, p, d_vec, d_out, d_k);
+}
 
 }  // namespace ggks
 
@@ -654,6 +778,13 @@ GGKS_TYPED_TOPK(ggks_topk_f64, double, GGKS_F64)
 GGKS_TYPED_TOPK(ggks_topk_u32, uint32_t, GGKS_U32)
 GGKS_TYPED_TOPK(ggks_topk_i32, int32_t, GGKS_I32)
 #undef GGKS_TYPED_TOPK
+
+int ggks_bucket_multiselect_noalloc(int dtype, const void* d_vec, int64_t n, const uint32_t* d_k,
+                                    int64_t num_k, void* d_out, const ggks_opts* opt,
+                                    void* stream) {
+  return select_noalloc(dtype, d_vec, n, d_k, num_k, d_out, opt, (hipStream_t)stream);
+}
+
 
 // ---------------------------------------------------------- plumbing
 int ggks_dev_alloc(void** ptr, size_t bytes) {

@@ -148,6 +148,25 @@ int ggks_radix_multiselect_async(ggks_ctx *ctx, int dtype, const void *d_vec,
                                  void *d_out, const ggks_opts *opt);
 int ggks_topk_async(ggks_ctx *ctx, int dtype, const void *d_vec, int64_t n,
                     int64_t K, void *d_out, const ggks_opts *opt);
+/* No counterpart in the reference.  For callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): bucket multiselect as ggks_bucket_multiselect_async (device k
+ * list, device outputs, stream-ordered on `stream`, a hipStream_t or NULL),
+ * but with no context and no allocation of any kind on any call: its scratch
+ * is a static device array of GGKS_NOALLOC_SCRATCH_BYTES that every GPU the
+ * library runs on holds until the process ends, and its plan tables are
+ * written there by a kernel (no host staging buffers).  A select whose
+ * scratch exceeds that size is refused with GGKS_ENOMEM before anything is
+ * launched (with 1024 k: float32 / uint32 / int32 up to about 1.6e7 keys,
+ * float64 up to about 8e6).  Every call on one GPU shares that GPU's array, so calls that
+ * overlap on one GPU (other streams, other host threads) race: serialise
+ * them.  k values must lie in [1, n]; an out-of-range device k is clamped
+ * to 1 and not reported (there is no context to report it to). */
+#define GGKS_NOALLOC_SCRATCH_BYTES (64ull << 20)
+int ggks_bucket_multiselect_noalloc(int dtype, const void *d_vec, int64_t n,
+                                    const uint32_t *d_k, int64_t num_k,
+                                    void *d_out, const ggks_opts *opt,
+                                    void *stream);
 /* Grow the arena so that a call of `kind` with these sizes allocates
  * nothing. */
 int ggks_ctx_reserve(ggks_ctx *ctx, int kind, int dtype, int64_t n,
