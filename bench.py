@@ -16,8 +16,16 @@ for the dominant kernel (the larger of the two streaming passes, each of which
 reads n * 4 bytes): achieved = n * 4 B / its mean event duration in the timed
 region.
 cpu_baseline = the reference's sort-and-choose (its vendored Thrust 1.4, OpenMP
-backend, oracle/_ref) on the host cores, over the first 2^24 elements of the
-same vector with 1024 k values, timed for a bounded ~10 s.
+backend, oracle/_ref) on the host cores over the whole 2^28 vector with the
+same 1024 k values (one run, ~20-30 s), with the single-core std::nth_element
+multiselect beside it and the host's CPU model.
+
+Further legs, each its own JSON object: f64 (2^27 float64), topk (K = 4096 of
+the headline vector), radix (radix multiselect of the headline vector and k),
+adversarial (2^27 u32 in [0,1023], Cauchy f32 and 50 %-0.5 plateau f32 with
+256 clustered k plus {1, n}, both paths, against 2^27 f32 N(0,1)), batched
+(4096 uneven vectors, batch against loop), serving (Poisson traces) and
+saturation (the highest offered rate the server keeps up with).
 """
 from __future__ import annotations
 
@@ -50,7 +58,6 @@ def parse():
     ap.add_argument("--blocks", type=int, default=0, help="chunk count (0 = tuned)")
     ap.add_argument("--buckets-log2", type=int, default=0, help="bucket budget (0 = tuned)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-looped leg")
     ap.add_argument("--no-f64", action="store_true", help="skip the 2^27 float64 leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
@@ -58,6 +65,13 @@ def parse():
                     help="length of each served trace (the workload definition uses 60)")
     ap.add_argument("--serve-rates", default="2000,8000,32000")
     ap.add_argument("--batch-segments", type=int, default=4096)
+    ap.add_argument("--no-adversarial", action="store_true", help="skip the adversarial leg")
+    ap.add_argument("--no-topk", action="store_true", help="skip the top-K and radix legs")
+    ap.add_argument("--no-saturation", action="store_true", help="skip the saturation sweep")
+    ap.add_argument("--saturation-rates", default="64000,128000,256000,512000")
+    ap.add_argument("--saturation-seconds", type=float, default=2.0)
+    ap.add_argument("--p99-bound-ms", type=float, default=5.0,
+                    help="latency bound of the saturation sweep")
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the timed steps, copy the vector once with the library's plain "
                          "16-byte copy kernel (known-size reference for memory-traffic counters)")
@@ -75,43 +89,164 @@ def make_input(pkg, dtype: str, n: int):
     return W.int_range(3, n, -(2 ** 31), 2 ** 31 - 1)
 
 
-def cpu_baseline(pkg, x: np.ndarray, numk: int, budget_s: float):
-    """Reference sort-and-choose on host cores over a bounded sample."""
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def host_threads() -> int:
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def cpu_baseline(pkg, x: np.ndarray, k: np.ndarray, gpu_out: np.ndarray):
+    """The reference's sort-and-choose on the host cores over the whole
+    workload vector (one run; the sort is destructive, so the copy-in is
+    timed separately), and the single-core std::nth_element multiselect over
+    the same vector; both must return the GPU's values."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O  # checker / baseline only
-    ns = min(x.size, 1 << 24)
-    sample = np.ascontiguousarray(x[:ns])
-    ks = pkg.workloads.k_uniform(101, ns, numk)
-    threads = os.cpu_count() or 1
+    threads = host_threads()
     kind = "reference" if O.ref_available() else "port"
-    times, res = [], None
-    t_start = time.perf_counter()
-    while True:
-        t0 = time.perf_counter()
-        if kind == "reference":
-            res, _sort_ms = O.ref_sort_choose(sample, ks, threads=threads)
-        else:
-            res = O.parallel_sort_choose(sample, ks, threads=threads)
-        times.append(time.perf_counter() - t0)
-        if time.perf_counter() - t_start >= budget_s or len(times) >= 20:
-            break
-    t = float(np.median(times))
-    # the GPU path on the same sample must return the same values
-    d = pkg.DeviceArray.from_host(sample)
-    gpu = pkg.multiselect(d, ks)
-    u = np.uint64 if sample.dtype == np.float64 else np.uint32
-    parity = bool(np.array_equal(gpu.view(u), res.view(u)))
+    n = x.size
+    t0 = time.perf_counter()
+    if kind == "reference":
+        res, sort_ms = O.ref_sort_choose(x, k, threads=threads)
+    else:
+        res = O.parallel_sort_choose(x, k, threads=threads)
+        sort_ms = None
+    t = time.perf_counter() - t0
+    u = np.uint64 if x.dtype == np.float64 else np.uint32
+    parity = bool(np.array_equal(gpu_out.view(u), res.view(u)))
+    t0 = time.perf_counter()
+    nres = O.nth_multiselect(x, k)
+    tn = time.perf_counter() - t0
+    nth_parity = bool(np.array_equal(gpu_out.view(u), nres.view(u)))
     return {
-        "value": round(ns / t / 1e9, 4),
+        "value": round(n / t / 1e9, 4),
         "unit": "Gelem/s",
         "cores": threads,
         "kind": kind,
-        "sample": (f"first 2^{int(np.log2(ns))} elements of the workload vector, {ks.size} "
-                   f"uniform k; {'vendored Thrust 1.4 OMP sort' if kind == 'reference' else 'libstdc++ parallel sort'}"
-                   f" + gather at n-k, median of {len(times)} runs (copy-in included)"),
-        "ms": round(t * 1e3, 2),
+        "cpu_model": cpu_model(),
+        "sample": (f"the whole workload vector (n=2^{int(np.log2(n))} {x.dtype}, {k.size} k), one "
+                   f"run; {'vendored Thrust 1.4 OMP sort' if kind == 'reference' else 'libstdc++ parallel sort'}"
+                   " + gather at n-k; ms includes the copy-in (the sort is destructive)"),
+        "ms": round(t * 1e3, 1),
+        "sort_ms": round(sort_ms, 1) if sort_ms is not None else None,
         "gpu_parity": parity,
+        "nth_element": {"value": round(n / tn / 1e9, 4), "unit": "Gelem/s", "cores": 1,
+                        "ms": round(tn * 1e3, 1), "gpu_parity": nth_parity,
+                        "method": "std::nth_element over the sorted distinct k, narrowing the "
+                                  "range each step (copy-in included)"},
     }
+
+
+def timed_selects(pkg, L, d, n, dk, dout, steps, kind="bucket", K=0, ctx=None):
+    """Mean wall time (ms) of `steps` stream-ordered selects between two
+    synchronisations, after three warm-up calls."""
+    lib = L.lib()
+    ctx = ctx or pkg.Context(0)
+    code = pkg.api.dtype_code(d.dtype)
+
+    def one():
+        if K:
+            L.check(lib.ggks_topk_async(ctx.handle, code, d.ptr, n, K, dout.ptr, None))
+        else:
+            pkg.multiselect_async(d, dk, dout, ctx, kind=kind)
+    for _ in range(3):
+        one()
+    L.check(lib.ggks_sync())
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        one()
+    L.check(lib.ggks_sync())
+    return (time.perf_counter() - t0) * 1e3 / steps
+
+
+def topk_radix_legs(pkg, L, d, x, k, bucket_out, steps):
+    """Top-K (K = 4096) and radix multiselect of the headline vector."""
+    n = x.size
+    esz = x.dtype.itemsize
+    K = 4096
+    dout = pkg.DeviceArray(K, x.dtype)
+    ms = timed_selects(pkg, L, d, n, None, dout, steps, K=K)
+    got = dout.to_host()
+    part = np.partition(x, n - K)[n - K:]
+    want = np.sort(part)[::-1]
+    u = np.uint64 if esz == 8 else np.uint32
+    topk = {"workload": f"top-K, K={K}, n=2^{int(np.log2(n))} {x.dtype} N(0,1)",
+            "ms_per_step": round(ms, 4), "gelem_per_s": round(n / ms / 1e6, 2),
+            "hbm_frac_call": round(2 * n * esz / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "bit_exact": bool(np.array_equal(got.view(u), want.view(u)))}
+    dk = pkg.DeviceArray.from_host(k)
+    dout = pkg.DeviceArray(k.size, x.dtype)
+    ms = timed_selects(pkg, L, d, n, dk, dout, steps, kind="radix")
+    got = dout.to_host()
+    radix = {"workload": f"radix multiselect, n=2^{int(np.log2(n))} {x.dtype} N(0,1), {k.size} uniform k",
+             "ms_per_step": round(ms, 4), "gelem_per_s": round(n / ms / 1e6, 2),
+             "hbm_frac_call": round(2 * n * esz / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+             "identical_to_bucket": bool(np.array_equal(got.view(u), bucket_out.view(u)))}
+    return topk, radix
+
+
+def adversarial_leg(pkg, L, steps):
+    """The adversarial configuration at n = 2^27: u32 uniform in [0,1023],
+    f32 Cauchy(0,1) and f32 plateau (0.5 with probability 1/2, else U[0,1)),
+    each with 256 distinct k clustered around n/2 plus k = 1 and k = n, on
+    the bucket and radix paths; f32 N(0,1) with the same k beside them.
+    Checks: both paths agree; closed forms from the histogram (u32), the
+    0.5 band (plateau) and the extremes (every input)."""
+    W = pkg.workloads
+    n = 1 << 27
+    k = W.k_cluster(103, n)
+    dk = pkg.DeviceArray.from_host(k)
+    ctx = pkg.Context(0)
+    out = []
+    for name, dt, fam, kw in (("u32 uniform [0,1023]", np.uint32, "uniform", {"lo": 0, "hi": 1023}),
+                              ("f32 Cauchy(0,1)", np.float32, "cauchy", {}),
+                              ("f32 plateau 0.5 (p=1/2) else U[0,1)", np.float32, "plateau", {}),
+                              ("f32 N(0,1) (comparison)", np.float32, "normal", {})):
+        print(f"# adversarial: {name}", file=sys.stderr, flush=True)
+        d = pkg.generate(dt, fam, 3 + len(out), n, **kw)
+        x = d.to_host()
+        dout = pkg.DeviceArray(k.size, dt)
+        rec = {"input": name, "n": n, "num_k": int(k.size)}
+        res = {}
+        for kind in ("bucket", "radix"):
+            ms = timed_selects(pkg, L, d, n, dk, dout, steps, kind=kind, ctx=ctx)
+            res[kind] = dout.to_host()
+            rec[kind] = {"ms_per_step": round(ms, 4), "gelem_per_s": round(n / ms / 1e6, 2),
+                         "hbm_frac_call": round(2 * n * 4 / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        ok = bool(np.array_equal(res["bucket"].view(np.uint32), res["radix"].view(np.uint32)))
+        ok &= bool(res["bucket"][-2] == x.max() and res["bucket"][-1] == x.min())
+        if dt == np.uint32:  # the k-th largest from the value histogram
+            desc = np.cumsum(np.bincount(x, minlength=1024)[::-1])
+            want = (1023 - np.searchsorted(desc, k.astype(np.int64), side="left")).astype(np.uint32)
+            ok &= bool(np.array_equal(res["bucket"], want))
+        elif fam == "plateau":  # ascending ranks inside the 0.5 band are 0.5
+            lt = int(np.count_nonzero(x < 0.5))
+            eq = int(np.count_nonzero(x == 0.5))
+            r = n - k.astype(np.int64) + 1
+            band = (r > lt) & (r <= lt + eq)
+            ok &= bool(band[:-2].all() and np.all(res["bucket"][band] == np.float32(0.5)))
+        rec["bit_exact_checks"] = ok
+        out.append(rec)
+        d.free()
+        del x
+    base = out[-1]["bucket"]["ms_per_step"]
+    for rec in out[:-1]:
+        for kind in ("bucket", "radix"):
+            rec[kind]["vs_normal_bucket"] = round(rec[kind]["ms_per_step"] / base, 2)
+    return {"workload": "n=2^27, 256 distinct k in [n/2-500, n/2+499] plus k=1 and k=n",
+            "inputs": out}
 
 
 def batched_leg(pkg, L, nseg: int):
@@ -256,6 +391,31 @@ def serving_leg(pkg, rates, seconds: float):
             "pool_elements": int(pool["lengths"].sum()), "rates": out}
 
 
+def saturation_leg(pkg, rates, seconds: float, p99_bound_ms: float):
+    """Offered load against completion: short Poisson traces at rising rates
+    over the serving pool; a rate is sustained when every request completes,
+    the trace drains within 10 % of its length and p99 stays under the
+    bound.  Reports the highest sustained rate."""
+    S = pkg.serving
+    pool = S.make_pool(5000, 64)
+    bufs = S.upload_pool(pool)
+    host = S.sorted_copies(pool, bufs)
+    srv = pkg.Server(bufs, scratch_budget=32 << 30, max_inflight=4, max_batch=256)
+    out, best = [], None
+    for j, rate in enumerate(rates):
+        r = S.run(srv, pool, host, rate, seconds, seed=7000 + j)
+        kept = (r["completed"] == r["requests"] and r["wall_s"] <= 1.1 * seconds
+                and r["p99_ms"] is not None and r["p99_ms"] <= p99_bound_ms and r["bit_exact"])
+        r["sustained"] = bool(kept)
+        print(f"# saturation {rate:.0f} req/s: {r}", file=sys.stderr, flush=True)
+        out.append(r)
+        if kept:
+            best = rate
+    srv.close()
+    return {"p99_bound_ms": p99_bound_ms, "trace_seconds": seconds,
+            "max_sustained_rate_per_s": best, "rates": out}
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -385,9 +545,16 @@ def main():
         },
         "cpu_baseline": None,
     }
+    if not a.no_topk:
+        print("# top-K and radix legs", file=sys.stderr, flush=True)
+        out["topk"], out["radix"] = topk_radix_legs(pkg, L, d, x, k, last, a.steps)
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(pkg, x, a.numk, a.cpu_seconds)
+        print("# cpu baseline", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(pkg, x, k, last)
     del d, x
+    if not a.no_adversarial:
+        print("# adversarial leg", file=sys.stderr, flush=True)
+        out["adversarial"] = adversarial_leg(pkg, L, a.steps)
     if not a.no_f64:
         print("# f64 leg", file=sys.stderr, flush=True)
         out["f64"] = f64_leg(pkg, L, a.steps)
@@ -398,6 +565,10 @@ def main():
         print("# serving leg", file=sys.stderr, flush=True)
         out["serving"] = serving_leg(pkg, [float(r) for r in a.serve_rates.split(",")],
                                      a.serve_seconds)
+    if not a.no_saturation:
+        print("# saturation leg", file=sys.stderr, flush=True)
+        out["saturation"] = saturation_leg(pkg, [float(r) for r in a.saturation_rates.split(",")],
+                                           a.saturation_seconds, a.p99_bound_ms)
     print(json.dumps(out), flush=True)
 
 

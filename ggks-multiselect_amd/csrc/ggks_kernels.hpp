@@ -201,8 +201,8 @@ __device__ __forceinline__ int bitlen(KT v) {
 // above the last prefix land in the first or last bucket.
 template <typename KT>
 struct BucketFn {
-  KT kb, lmax;
-  uint32_t poff, pwid, rsh, twid, lsh;
+  KT kb, lmax, hv;
+  uint32_t poff, pwid, rsh, twid, lsh, hvon;
   __device__ __forceinline__ void init(const SegState& st) {
     kb = (KT)st.kb;
     lmax = (KT)st.lmax;
@@ -211,6 +211,12 @@ struct BucketFn {
     rsh = st.rsh;
     twid = st.twid;
     lsh = st.lsh;
+    hv = (KT)st.hv;
+    hvon = st.hvon;
+  }
+  // F from L for a segment with a heavy value: + [key >= hv] + [key > hv]
+  __device__ __forceinline__ uint32_t heavy(uint32_t b, KT key) const {
+    return b + (key >= hv ? 1u : 0u) + (key > hv ? 1u : 0u);
   }
   // stage 1: LUT index p and the normalised sub-field t of a key
   __device__ __forceinline__ void split(KT key, uint32_t& p, uint32_t& t) const {
@@ -251,7 +257,8 @@ struct BucketFn {
   __device__ __forceinline__ uint32_t operator()(KT key, const uint32_t* lut) const {
     uint32_t p, t;
     split(key, p, t);
-    return finish(lut[p], t);
+    const uint32_t b = finish(lut[p], t);
+    return hvon ? heavy(b, key) : b;
   }
 };
 
@@ -273,12 +280,17 @@ __device__ __forceinline__ void bucket_batch(const BucketFn<KT>& fn, const uint3
   for (int i = 0; i < N; ++i) e[i] = lut[p[i]];
 #pragma unroll
   for (int i = 0; i < N; ++i) b[i] = fn.finish(e[i], t[i]);
+  if (fn.hvon) {  // uniform
+#pragma unroll
+    for (int i = 0; i < N; ++i) b[i] = fn.heavy(b[i], to_key<C>(raw[i]));
+  }
 }
 
 // bucket_batch for the streaming passes: split1 with the shift direction
-// fixed at compile time (each pass runs its loop in the instance chosen by
-// the segment's s, so the loop body has no branch).
-template <typename KT, int C, int N, bool LEFT>
+// and the heavy-value term fixed at compile time (each pass runs its loop in
+// the instance chosen by the segment's state, so the loop body has no
+// branch).
+template <typename KT, int C, int N, bool LEFT, bool HV = false>
 __device__ __forceinline__ void bucket_batch1(const BucketFn<KT>& fn, const uint32_t* lut,
                                               const KT (&raw)[N], uint32_t (&b)[N]) {
   uint32_t p[N], t[N], e[N];
@@ -288,6 +300,10 @@ __device__ __forceinline__ void bucket_batch1(const BucketFn<KT>& fn, const uint
   for (int i = 0; i < N; ++i) e[i] = lut[p[i]];
 #pragma unroll
   for (int i = 0; i < N; ++i) b[i] = fn.finish(e[i], t[i]);
+  if (HV) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) b[i] = fn.heavy(b[i], to_key<C>(raw[i]));
+  }
 }
 
 // Streaming 16-byte load of input data: non-temporal, so the read stream
@@ -320,10 +336,26 @@ struct RangeOf {
   BucketFn<KT> fn;
   const uint32_t* lut;
   uint32_t B, P, s;
-  // Closed-form candidate for the smallest key of bucket b: find the prefix p
-  // holding b, then the smallest t with floor(t * nb / 2^24) >= j and the
-  // smallest sub-field value giving that t.
+  uint32_t hb;  // F's bucket of the heavy value (fn.hvon)
+  // Closed-form candidate for the smallest key of F-bucket b: with a heavy
+  // value, buckets up to hb - 1 are L's, hb is {hv}, hb + 1 starts at
+  // hv + 1 and the rest are L's shifted by two.
   __device__ bool candidate(uint32_t b, KT& out) const {
+    if (fn.hvon) {
+      if (b == hb) { out = fn.hv; return true; }
+      if (b == hb + 1) {
+        if (fn.hv == KeyBits<KT>::MAX) return false;
+        out = (KT)(fn.hv + 1);
+        return true;
+      }
+      if (b > hb + 1) b -= 2;
+    }
+    return candidate_l(b, out);
+  }
+  // Closed-form candidate for the smallest key of L-bucket b: find the
+  // prefix p holding b, then the smallest t with floor(t * nb / 2^24) >= j
+  // and the smallest sub-field value giving that t.
+  __device__ bool candidate_l(uint32_t b, KT& out) const {
     uint32_t lo = 0, hi = P - 1;  // smallest prefix whose bucket range ends past b
     while (lo < hi) {
       uint32_t mid = (lo + hi) >> 1;
@@ -427,21 +459,27 @@ __global__ void k_topk_queries(uint32_t K, uint64_t n, Query* __restrict__ q) {
 // Seeded random sample of each segment: one work-group per block of 1024
 // samples of a segment (the planner's dense table; every sample count is a
 // multiple of 1024), kSampleThreads threads of kSamplePer independent
-// gathers each.  Each work-group also writes the min and max key of its 1024
-// samples, so the LUT kernel reads 64 pairs instead of scanning the sample
-// for its range.
+// gathers each.  Each work-group also writes a record of four keys for its
+// 1024 samples: their min and max (so the LUT kernel reads 64 records
+// instead of scanning the sample for its range), and a heavy-value
+// candidate with its count: every wavefront proposes its first sample, the
+// block counts each proposal's copies among its 1024 samples, and keeps the
+// most frequent (the smaller key on a tie).
 constexpr uint32_t kSampleThreads = 256;
 constexpr uint32_t kSamplePer = kThreads / kSampleThreads;
+constexpr uint32_t kSampleWaves = kSampleThreads / 64;
 template <typename KT, int C>
 __global__ __launch_bounds__(kSampleThreads) void k_sample(const KT* __restrict__ src,
                                                            const SegDesc* __restrict__ segs,
                                                            const SampleBlock* __restrict__ sblocks,
                                                            KT* __restrict__ samples,
                                                            KT* __restrict__ minmax, uint64_t seed) {
-  __shared__ KT red_lo[kSampleThreads / 64], red_hi[kSampleThreads / 64];
+  __shared__ KT red_lo[kSampleWaves], red_hi[kSampleWaves], cand[kSampleWaves];
+  __shared__ uint32_t ccnt[kSampleWaves][kSampleWaves];  // [wave][candidate]
   const SampleBlock sb = sblocks[blockIdx.x];
   const SegDesc& sd = segs[sb.seg];
-  if (sd.mode != MODE_SAMPLE || sb.blk * kThreads >= sd.nsample) return;  // uniform
+  if (sd.mode == MODE_RANGE || sb.blk * kThreads >= sd.nsample) return;  // uniform
+  const uint32_t w = threadIdx.x >> 6, lane = lane_id();
   KT k[kSamplePer];
 #pragma unroll
   for (uint32_t j = 0; j < kSamplePer; ++j) {  // sample i of the segment
@@ -456,32 +494,71 @@ __global__ __launch_bounds__(kSampleThreads) void k_sample(const KT* __restrict_
 #pragma unroll
   for (uint32_t j = 0; j < kSamplePer; ++j) {
     const KT x = to_key<C>(k[j]);
+    k[j] = x;
     samples[sd.sample_off + sb.blk * kThreads + j * kSampleThreads + threadIdx.x] = x;
     lo = x < lo ? x : lo;
     hi = x > hi ? x : hi;
   }
+  if (lane == 0) cand[w] = k[0];
   for (int d = 32; d >= 1; d >>= 1) {
     const KT a = __shfl_xor(lo, d, 64), b = __shfl_xor(hi, d, 64);
     lo = a < lo ? a : lo;
     hi = b > hi ? b : hi;
   }
-  if (lane_id() == 0) {
-    red_lo[threadIdx.x >> 6] = lo;
-    red_hi[threadIdx.x >> 6] = hi;
+  if (lane == 0) {
+    red_lo[w] = lo;
+    red_hi[w] = hi;
+  }
+  __syncthreads();
+#pragma unroll
+  for (uint32_t c = 0; c < kSampleWaves; ++c) {
+    const KT cv = cand[c];
+    uint32_t m = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < kSamplePer; ++j) m += k[j] == cv ? 1u : 0u;
+    m = wave_last(wave_incl_scan(m));
+    if (lane == 0) ccnt[w][c] = m;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    for (uint32_t w = 1; w < kSampleThreads / 64; ++w) {
-      lo = red_lo[w] < lo ? red_lo[w] : lo;
-      hi = red_hi[w] > hi ? red_hi[w] : hi;
+    for (uint32_t v = 1; v < kSampleWaves; ++v) {
+      lo = red_lo[v] < lo ? red_lo[v] : lo;
+      hi = red_hi[v] > hi ? red_hi[v] : hi;
     }
-    KT* mm = minmax + 2 * (sd.sample_off / kThreads + sb.blk);
+    KT best = cand[0];
+    uint32_t bn = 0;
+    for (uint32_t c = 0; c < kSampleWaves; ++c) {
+      uint32_t t = 0;
+      for (uint32_t v = 0; v < kSampleWaves; ++v) t += ccnt[v][c];
+      if (t > bn || (t == bn && cand[c] < best)) {
+        bn = t;
+        best = cand[c];
+      }
+    }
+    KT* mm = minmax + 4 * (sd.sample_off / kThreads + sb.blk);
     mm[0] = lo;
     mm[1] = hi;
+    mm[2] = best;
+    mm[3] = (KT)bn;
   }
 }
 
+// A sampled key is a heavy value when it is at least this many of one
+// block's kThreads samples (3.1 %): it then gets a bucket of its own.
+constexpr uint32_t kHeavyMinCount = 32;
+
 // Build one segment's bucket function (one block per segment).
+//  MODE_SAMPLE  prefixes of 2^s keys over the sampled range, sub-buckets in
+//               proportion to each prefix's share of the sample; a run of
+//               empty prefixes shares one bucket.
+//  MODE_RADIX   MSD radix digits: the digit is the next radix_bits bits below
+//               the longest common prefix of the sampled keys (the digit
+//               passes the reference skips when every key shares the digit,
+//               RadixSelect_api.h:590-592, are skipped up front), each
+//               digit value exactly its buckets; keys outside the sampled
+//               window fall into a guard bucket below or above every digit.
+//  MODE_RANGE   equal-width buckets over the segment's known key range.
+// A heavy value of the sample (SegState) takes two more buckets.
 template <typename KT>
 __global__ __launch_bounds__(kThreads) void k_lut(const SegDesc* __restrict__ segs,
                                                   const KT* __restrict__ samples,
@@ -491,47 +568,76 @@ __global__ __launch_bounds__(kThreads) void k_lut(const SegDesc* __restrict__ se
   __shared__ uint32_t cnt[1u << kMaxPrefixBits];
   __shared__ uint32_t wsum[17];
   __shared__ KT red_lo[16], red_hi[16];
+  __shared__ KT sh_hv;
+  __shared__ uint32_t sh_hvon;
   constexpr uint32_t W = KeyBits<KT>::W;
   const SegDesc sd = segs[blockIdx.x];
   const uint32_t tid = threadIdx.x;
   uint32_t* lut = lut_pool + sd.lut_off;
   KT kmin, kmax;
   uint32_t pb = sd.pbits;
+  uint32_t hvon = 0;
+  KT hv = 0;
 
-  if (sd.mode == MODE_SAMPLE) {
-    KT lo = KeyBits<KT>::MAX, hi = 0;
-    const uint32_t npairs = sd.nsample / kThreads;  // <= 64
+  if (sd.mode == MODE_SAMPLE || sd.mode == MODE_RADIX) {
+    KT lo = KeyBits<KT>::MAX, hi = 0, hc = 0;
+    uint32_t hn = 0;
+    const uint32_t npairs = sd.nsample / kThreads;  // <= 64: wavefront 0 holds them all
     if (tid < npairs) {
-      const KT* mm = minmax + 2 * (sd.sample_off / kThreads + tid);
+      const KT* mm = minmax + 4 * (sd.sample_off / kThreads + tid);
       lo = mm[0];
       hi = mm[1];
+      hc = mm[2];
+      hn = (uint32_t)mm[3];
     }
     for (int d = 32; d >= 1; d >>= 1) {
       KT a = __shfl_xor(lo, d, 64), b = __shfl_xor(hi, d, 64);
       lo = a < lo ? a : lo;
       hi = b > hi ? b : hi;
+      const KT oc = __shfl_xor(hc, d, 64);
+      const uint32_t on = (uint32_t)__shfl_xor((int)hn, d, 64);
+      if (on > hn || (on == hn && oc < hc)) {
+        hn = on;
+        hc = oc;
+      }
     }
     if (lane_id() == 0) { red_lo[tid >> 6] = lo; red_hi[tid >> 6] = hi; }
+    if (tid == 0) {
+      sh_hvon = hn >= kHeavyMinCount ? 1u : 0u;
+      sh_hv = hc;
+    }
     __syncthreads();
     kmin = red_lo[0]; kmax = red_hi[0];
     for (int w = 1; w < 16; ++w) {
       kmin = red_lo[w] < kmin ? red_lo[w] : kmin;
       kmax = red_hi[w] > kmax ? red_hi[w] : kmax;
     }
-  } else if (sd.mode == MODE_RADIX) {
-    kmin = 0;
-    kmax = KeyBits<KT>::MAX;
+    hvon = sh_hvon;
+    hv = sh_hv;
   } else {
     kmin = (KT)sd.lo;
     kmax = (KT)sd.hi;
   }
   const KT R = kmax - kmin;
   const int bl = bitlen<KT>(R);
+  KT kb = kmin;
   uint32_t s, P;
+  // radix: digit window, guard prefixes (below / above the window), digit
+  // prefixes of 2^s keys holding nbd exact sub-digits each
+  uint32_t glo = 0, ndp = 0, nbd = 1;
   if (sd.mode == MODE_RADIX) {
-    pb = sd.radix_bits < pb ? sd.radix_bits : pb;
-    s = W - pb;
-    P = 1u << pb;
+    const uint32_t D = (uint32_t)bitlen<KT>(kmin ^ kmax);  // low bits the sampled keys differ in
+    const uint32_t db = D < sd.radix_bits ? D : sd.radix_bits;
+    const uint32_t dpb = db < kMaxPrefixBits - 1 ? db : kMaxPrefixBits - 1;
+    s = D - dpb;
+    nbd = 1u << (db - dpb);
+    ndp = 1u << dpb;
+    const KT base = D >= W ? KT(0) : (KT)((kmin >> D) << D);  // common prefix of the window
+    const KT top = D >= W ? KeyBits<KT>::MAX : (KT)(base + ((KT(1) << D) - 1));
+    glo = base > 0 ? 1u : 0u;
+    kb = (KT)(base - ((KT)glo << s));
+    P = glo + ndp + (top < KeyBits<KT>::MAX ? 1u : 0u);
+    pb = (uint32_t)bitlen<uint32_t>(P - 1);
   } else if (sd.mode == MODE_RANGE) {
     pb = 0;
     s = (uint32_t)bl;
@@ -540,15 +646,17 @@ __global__ __launch_bounds__(kThreads) void k_lut(const SegDesc* __restrict__ se
     s = bl > (int)pb ? (uint32_t)(bl - (int)pb) : 0u;
     P = (uint32_t)(((uint64_t)R >> s) + 1);  // <= 2^pb
   }
-  const uint32_t Bt = sd.bmax;  // bucket budget
+  const uint32_t Bt = sd.bmax - (hvon ? 2u : 0u);  // bucket budget of L
   const uint32_t cap = s >= 16 ? 65535u : min(65535u, 1u << s);
   uint32_t Btot;
 
   if (sd.mode == MODE_RADIX) {
-    // every prefix gets 2^(radix_bits - pb) buckets: exactly the next bits
-    const uint32_t nb = 1u << (sd.radix_bits - pb);
-    for (uint32_t p = tid; p < P; p += kThreads) lut[p] = (p * nb) | (nb << 16);
-    Btot = P * nb;
+    for (uint32_t p = tid; p < P; p += kThreads) {
+      if (p < glo) lut[p] = 0u | (1u << 16);
+      else if (p < glo + ndp) lut[p] = (glo + (p - glo) * nbd) | (nbd << 16);
+      else lut[p] = (glo + ndp * nbd) | (1u << 16);
+    }
+    Btot = glo + ndp * nbd + (P - glo - ndp);
   } else if (sd.mode == MODE_RANGE) {
     const uint32_t nb = min(Bt, cap);
     if (tid == 0) lut[0] = nb << 16;
@@ -592,13 +700,14 @@ __global__ __launch_bounds__(kThreads) void k_lut(const SegDesc* __restrict__ se
     }
     Btot = tot;
   }
+  __syncthreads();  // the LUT is complete (thread 0 reads hv's entry)
   if (tid == 0) {
     SegState st = {};
-    st.kb = (uint64_t)kmin;
+    st.kb = (uint64_t)kb;
     st.kmax = (uint64_t)kmax;
     // lmax = min(P * 2^s - 1, KMAX - kb): every key maps into a prefix < P
     const unsigned __int128 span = ((unsigned __int128)P << s) - 1;
-    const unsigned __int128 room = (unsigned __int128)(KeyBits<KT>::MAX - kmin);
+    const unsigned __int128 room = (unsigned __int128)(KeyBits<KT>::MAX - kb);
     st.lmax = (uint64_t)(span < room ? span : room);
     st.s = s;
     st.P = P;
@@ -609,6 +718,17 @@ __global__ __launch_bounds__(kThreads) void k_lut(const SegDesc* __restrict__ se
     st.twid = s < 24 ? s : 24;
     st.rsh = s > 24 ? s - 24 : 0;
     st.lsh = s < 24 ? 24 - s : 0;
+    st.hvon = 0;
+    st.hv = 0;
+    st.hb = 0;
+    if (hvon) {  // hv's bucket is L(hv) + 1; two more buckets in all
+      BucketFn<KT> fn;
+      fn.init(st);
+      st.hvon = 1;
+      st.hv = (uint64_t)hv;
+      st.hb = fn(hv, lut) + 1;
+      st.B = Btot + 2;
+    }
     states[blockIdx.x] = st;
   }
 }
@@ -650,7 +770,7 @@ __global__ __launch_bounds__(kThreads) void k_count(const KT* __restrict__ src,
   uint64_t r0 = 0;
   // full rounds, software pipelined: the next round's loads are in flight
   // while a round is bucketed and counted
-  auto full_rounds = [&](auto left) {
+  auto full_rounds = [&](auto left, auto hvt) {
     if (r0 + per_round > nvec) return;  // uniform
     uint4 v[R];
 #pragma unroll
@@ -664,13 +784,19 @@ __global__ __launch_bounds__(kThreads) void k_count(const KT* __restrict__ src,
           v[t] = stream_load(&pv[r0 + per_round + (uint64_t)t * kThreads + tid]);
       }
       uint32_t b[N];
-      bucket_batch1<KT, C, N, decltype(left)::value>(bk, lut, k, b);
+      bucket_batch1<KT, C, N, decltype(left)::value, decltype(hvt)::value>(bk, lut, k, b);
 #pragma unroll
       for (int i = 0; i < N; ++i) atomicAdd(&hist[b[i]], 1u);
     }
   };
-  if (st.s <= 24) full_rounds(std::true_type{});  // uniform: the sub-field shift direction
-  else full_rounds(std::false_type{});
+  // uniform: the sub-field shift direction and the heavy-value term
+  if (!st.hvon) {
+    if (st.s <= 24) full_rounds(std::true_type{}, std::false_type{});
+    else full_rounds(std::false_type{}, std::false_type{});
+  } else {
+    if (st.s <= 24) full_rounds(std::true_type{}, std::true_type{});
+    else full_rounds(std::false_type{}, std::true_type{});
+  }
   for (uint64_t i = r0 + tid; i < nvec; i += kThreads) {  // last partial round
     uint4 v[1] = {stream_load(&pv[i])};
     KT k[VEC];
@@ -857,6 +983,7 @@ __device__ __forceinline__ void findk_segment(const SegDesc* __restrict__ segs,
   range.B = B;
   range.P = st.P;
   range.s = st.s;
+  range.hb = st.hb;
   auto count_of = [&](uint32_t b) { return (b + 1 < B ? hist[b + 1] : len) - hist[b]; };
 
   // A: bucket of every rank; single-key buckets are answered here
@@ -871,9 +998,10 @@ __device__ __forceinline__ void findk_segment(const SegDesc* __restrict__ segs,
     }
     // a bucket inside a prefix of 2^s keys split nb < 2^16 ways spans at
     // least floor(2^s / nb) >= 2 keys when s >= 17 (edge buckets only
-    // widen), so its exact range is needed only for smaller prefixes
+    // widen), so its exact range is needed only for smaller prefixes, and
+    // for the heavy value's bucket (one key by construction)
     KT klo = 0, khi = 1;
-    if (st.s < 17) range(lo, klo, khi);
+    if (st.s < 17 || (st.hvon && lo == st.hb)) range(lo, klo, khi);
     if (klo == khi) {
       out[q.out] = from_key<C>(klo);
       qb[sd.q_off + i] = NONE;
@@ -1174,12 +1302,12 @@ __device__ __forceinline__ void compact_chunk(const KT* __restrict__ src,
     }
   };
   if (rounds) load_round(0);
-  auto main_loop = [&](auto left) {
+  auto main_loop = [&](auto left, auto hvt) {
     for (uint64_t r = 0; r < rounds; ++r) {
       KT k[N];
       unpack<KT, kRound>(v, k);
       uint32_t b[N], act[N];
-      bucket_batch1<KT, C, N, decltype(left)::value>(bk, lut, k, b);
+      bucket_batch1<KT, C, N, decltype(left)::value, decltype(hvt)::value>(bk, lut, k, b);
 #pragma unroll
       for (int i = 0; i < N; ++i) act[i] = act_of[b[i]];
       if (r >= full_rounds) {  // uniform: the chunk's last, partial round
@@ -1207,8 +1335,14 @@ __device__ __forceinline__ void compact_chunk(const KT* __restrict__ src,
       if (nb > FLUSH_AT) flush(nb);  // uniform
     }
   };
-  if (st.s <= 24) main_loop(std::true_type{});  // uniform: the sub-field shift direction
-  else main_loop(std::false_type{});
+  // uniform: the sub-field shift direction and the heavy-value term
+  if (!st.hvon) {
+    if (st.s <= 24) main_loop(std::true_type{}, std::false_type{});
+    else main_loop(std::false_type{}, std::false_type{});
+  } else {
+    if (st.s <= 24) main_loop(std::true_type{}, std::true_type{});
+    else main_loop(std::false_type{}, std::true_type{});
+  }
   const uint32_t nb = nbuf;  // stable: nothing appends any more
   __syncthreads();
   flush(nb);
@@ -1716,10 +1850,14 @@ __global__ __launch_bounds__(kThreads) void k_finish(const KT* __restrict__ inpu
       }
       if (c.need_range) {
         KT a = KeyBits<KT>::MAX, b = 0;
-        for (uint32_t i = tid; i < c.len; i += kThreads) {
-          KT k = inl ? keys[i] : to_key<C>(srcp[i]);
+        auto mm = [&](KT k) {
           a = k < a ? k : a;
           b = k > b ? k : b;
+        };
+        if (inl) {
+          for (uint32_t i = tid; i < c.len; i += kThreads) mm(keys[i]);
+        } else {  // 16-byte loads, four in flight per thread
+          for_each_elem(srcp, c.len, [&](KT raw) { mm(to_key<C>(raw)); });
         }
         for (int d = 32; d >= 1; d >>= 1) {
           KT x = __shfl_xor(a, d, 64), y = __shfl_xor(b, d, 64);
@@ -1814,8 +1952,8 @@ __global__ __launch_bounds__(kThreads) void k_finish(const KT* __restrict__ inpu
             };
             if (inl) {
               for (uint32_t i = tid; i < c.len; i += kThreads) visit(keys[i]);
-            } else {
-              for (uint64_t i = tid; i < c.len; i += kThreads) visit(to_key<C>(srcp[i]) - lo);
+            } else {  // 16-byte loads, four in flight per thread
+              for_each_elem(srcp, c.len, [&](KT raw) { visit((KT)(to_key<C>(raw) - lo)); });
             }
             __syncthreads();
             // exclusive scan of each group's 256 bins: one wave per group

@@ -40,6 +40,10 @@ inline Tuning default_tuning(uint64_t n, int eb) {
 // in a batch stay multiples of 256 buckets.
 inline uint32_t radix_budget_log2(uint32_t rb) { return std::max<uint32_t>(rb + 1, 8); }
 
+// LUT prefix bits of a radix segment: up to 2^min(rb, 11) digit prefixes and
+// two guard prefixes (keys below / above the sampled digit window).
+inline uint32_t radix_lut_bits(uint32_t rb) { return std::min<uint32_t>(kMaxPrefixBits, rb + 1); }
+
 struct Plan {
   std::vector<SegDesc> segs;
   std::vector<Chunk> chunks;
@@ -115,7 +119,7 @@ inline SingleShape single_shape(int kind, int dtype, uint64_t n, const ggks_opts
     uint32_t rb = o.radix_bits ? (uint32_t)o.radix_bits : 14u;
     sh.rb = std::min<uint32_t>(14, std::max<uint32_t>(4, rb));
     bl = radix_budget_log2(sh.rb);
-    sh.pb = std::min<uint32_t>(kMaxPrefixBits, sh.rb);
+    sh.pb = radix_lut_bits(sh.rb);
   } else {
     bl = o.buckets_log2 ? (uint32_t)o.buckets_log2 : t.buckets_log2;
     bl = std::min<uint32_t>(15, std::max<uint32_t>(8, bl));
@@ -129,7 +133,7 @@ inline SingleShape single_shape(int kind, int dtype, uint64_t n, const ggks_opts
   csz = (csz + 63) / 64 * 64;
   sh.G = (n + csz - 1) / csz;
   sh.csz = csz;
-  if (sh.mode == MODE_SAMPLE) {
+  {  // both modes sample: the LUT (bucket) or the digit window (radix)
     uint64_t want = (n / 256 + 1023) / 1024 * 1024;
     sh.ns = (uint32_t)std::min<uint64_t>(65536, std::max<uint64_t>(4096, want));
   }
@@ -236,7 +240,7 @@ inline void plan_single(Plan& p, int kind, int dtype, uint64_t n, uint32_t nq,
   p.tables_end = off;
   p.off_states = take(sizeof(SegState));
   p.off_samples = take((size_t)std::max<uint32_t>(ns, 1) * eb);
-  p.off_minmax = take((size_t)std::max<uint32_t>(ns / 1024, 1) * 2 * eb);
+  p.off_minmax = take((size_t)std::max<uint32_t>(ns / 1024, 1) * 4 * eb);
   p.off_lut = take((size_t)(1u << pb) * 4);
   p.off_rows = take((size_t)G * bmax * 4);
   p.off_tot = take((size_t)bmax * 4);
@@ -364,7 +368,7 @@ inline bool plan_batch(Plan& p, int dtype, const std::vector<BatchSeg>& in, cons
       rb = o.radix_bits ? (uint32_t)o.radix_bits : 14u;
       rb = std::min<uint32_t>(14, std::max<uint32_t>(4, rb));
       bl = radix_budget_log2(rb);
-      pb = std::min<uint32_t>(kMaxPrefixBits, rb);
+      pb = radix_lut_bits(rb);
     } else {
       bl = o.buckets_log2 ? (uint32_t)o.buckets_log2 : (bt.buckets_log2 ? bt.buckets_log2 : 12u);
       bl = std::min<uint32_t>(15, std::max<uint32_t>(8, bl));
@@ -382,7 +386,7 @@ inline bool plan_batch(Plan& p, int dtype, const std::vector<BatchSeg>& in, cons
     // gather that fetches a whole line, and a batch of many segments pays
     // for them in HBM bandwidth; sparser samples leave buckets a little less
     // even, which costs less than the gathers save
-    if (mode == MODE_SAMPLE) {
+    {  // both modes sample: the LUT (bucket) or the digit window (radix)
       uint64_t want = (n / 1024 + 1023) / 1024 * 1024;
       ns = (uint32_t)std::min<uint64_t>(65536, std::max<uint64_t>(1024, want));
     }
@@ -468,7 +472,7 @@ inline bool plan_batch(Plan& p, int dtype, const std::vector<BatchSeg>& in, cons
   p.tables_end = off;
   p.off_states = take(nseg * sizeof(SegState));
   p.off_samples = take((size_t)std::max<uint64_t>(sam, 1) * eb);
-  p.off_minmax = take((size_t)std::max<uint64_t>(sam / 1024, 1) * 2 * eb);
+  p.off_minmax = take((size_t)std::max<uint64_t>(sam / 1024, 1) * 4 * eb);
   p.off_lut = take((size_t)lutc * 4);
   p.off_rows = take((size_t)rowc * 4);
   p.off_tot = take((size_t)totc * 4);

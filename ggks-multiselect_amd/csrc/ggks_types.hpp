@@ -71,16 +71,25 @@ struct SegDesc {          // planned on the host, read-only on the device
 //   t   = ((low >> rsh) & (2^twid - 1)) << lsh   the low s bits of low,
 //                                           normalised to 24 bits
 //   b   = base[p] + (t * nb[p]) >> 24       (one 24-bit multiply-high)
+//
+// A heavy value hv (a key covering a large share of the sample) gets a
+// bucket of its own: with L the function above, the bucket is
+//   F(key) = L(key) + [key >= hv] + [key > hv]
+// which is monotone (keys below hv keep L <= L(hv), hv maps to L(hv) + 1,
+// keys above to L + 2 >= L(hv) + 2), and bucket L(hv) + 1 holds hv alone.
 struct SegState {         // written on the device by the LUT kernel
   uint64_t kb;            // base key
   uint64_t kmax;          // largest sampled key (information)
   uint64_t lmax;          // clamp of key - kb
+  uint64_t hv;            // heavy value (when hvon)
   uint32_t s;             // prefix shift
   uint32_t P;             // number of prefixes
-  uint32_t B;             // number of buckets
+  uint32_t B;             // number of buckets (F's when hvon)
   uint32_t nact;          // active buckets (children)
   uint32_t poff, pwid;    // prefix bit field
   uint32_t rsh, twid, lsh;// sub-bucket bit field and normalising shift
+  uint32_t hvon;          // 1: the heavy-value bucket is in use
+  uint32_t hb;            // F's bucket of hv (L(hv) + 1) when hvon
   uint32_t pad;
 };
 

@@ -11,7 +11,11 @@
 - every generator family: device output against the host mirror where the
   two are bit-identical, and bucket / radix multiselect against the oracle
   on the read-back data with the reference's k families;
-- the Python server's tags: repeated caller tags each come back once.
+- the Python server's tags: repeated caller tags each come back once;
+- heavy values (a key repeated over 2 % .. 99.9 % of the vector: its own
+  bucket) at the middle, smallest and largest keys, single and batched;
+- the radix digit window below the sampled keys' common prefix, with keys
+  outside it in the guard buckets.
 """
 import os
 import subprocess
@@ -197,3 +201,88 @@ def test_server_repeated_tags(pkg):
         assert np.array_equal(out, srt[x.size - k.astype(np.int64)])
     assert srv.stats()["failed"] == 0
     srv.close()
+
+
+HEAVY_DT = [np.float32, np.float64, np.uint32, np.int32]
+
+
+def _heavy_input(W, dtype, n, seed, frac, where):
+    if dtype == np.float32:
+        x = W.normal_f32(seed, n)
+        v = {"mid": np.float32(0.25), "min": np.float32(-np.inf), "max": np.float32(np.inf)}[where]
+    elif dtype == np.float64:
+        x = W.uniform_f64(seed, n)
+        v = {"mid": 0.5, "min": -np.inf, "max": np.inf}[where]
+    elif dtype == np.uint32:
+        x = W.uint_range(seed, n, 0, 2**32 - 1)
+        v = {"mid": np.uint32(123456789), "min": np.uint32(0), "max": np.uint32(2**32 - 1)}[where]
+    else:
+        x = W.int_range(seed, n, -2**31, 2**31 - 1)
+        v = {"mid": np.int32(-77), "min": np.int32(-2**31), "max": np.int32(2**31 - 1)}[where]
+    m = int(round(frac * n))
+    pos = np.argsort(W.splitmix64(seed ^ 0x5151, 0, n), kind="stable")[:m]
+    x[pos] = v
+    return x
+
+
+@pytest.mark.parametrize("dtype", HEAVY_DT)
+def test_heavy_values(pkg, O, dtype):
+    """A value repeated at 2 % .. 99.9 % of the vector (its own bucket when the
+    sample shows it), in the middle of the range and at the smallest and
+    largest keys; ranks around it, at both ends and spread out."""
+    W = pkg.workloads
+    n = (1 << 22) + 5
+    for frac in (0.02, 0.05, 0.5, 0.999):
+        for where in ("mid", "min", "max"):
+            _say(f"heavy {np.dtype(dtype).name} frac={frac} at {where}")
+            x = _heavy_input(W, dtype, n, 31, frac, where)
+            k = np.concatenate([W.k_uniform(5, n, 200, dedup=False), W.k_cluster(6, n, 64, 400),
+                                [1, 2, n - 1, n]]).astype(np.uint32)
+            want = O.sort_choose(x, k)
+            d = pkg.DeviceArray.from_host(x)
+            for kind in ("bucket", "radix"):
+                got = pkg.multiselect(d, k, kind=kind)
+                bad = np.nonzero(bits(got) != bits(want))[0]
+                assert bad.size == 0, (frac, where, kind, bad[:4], got[bad[:4]], want[bad[:4]])
+            t = pkg.topk(d, 300)
+            assert np.array_equal(bits(t), bits(O.topk(x, 300))), (frac, where, "topk")
+            d.free()
+
+
+def test_heavy_values_batched(pkg, O):
+    """Segments with and without a heavy value in one batch, both kinds."""
+    W = pkg.workloads
+    lens = [(1 << 21) + 3, 300000, 1 << 20, 77777]
+    xs = [_heavy_input(W, np.float32, m, 40 + i, f, w)
+          for i, (m, f, w) in enumerate(zip(lens, (0.5, 0.0, 0.1, 0.3), ("mid", "mid", "max", "min")))]
+    ks = [np.concatenate([W.k_uniform(50 + i, m, 40, dedup=False), [1, m, m // 2]]).astype(np.uint32)
+          for i, m in enumerate(lens)]
+    for kind in ("bucket", "radix"):
+        table = pkg.segment_table(lens, [kind] * 4, [k.size for k in ks])
+        b = pkg.Batch(np.float32, table, np.concatenate(ks))
+        got = b(pkg.DeviceArray.from_host(np.concatenate(xs)))
+        want = np.concatenate([O.sort_choose(x, k) for x, k in zip(xs, ks)])
+        assert np.array_equal(bits(got), bits(want)), kind
+        b.close()
+
+
+@pytest.mark.parametrize("dtype", [np.uint32, np.float32])
+def test_radix_digit_window(pkg, O, dtype):
+    """The radix digit starts below the sampled keys' common prefix: narrow
+    windows anywhere in the key space, keys outside the sampled window (in
+    the guard buckets), and a window that is one key wide."""
+    W = pkg.workloads
+    n = (1 << 21) + 9
+    for lo, hi in ((0, 1023), (5000, 5000 + 70000), (2**31 - 10, 2**31 + 10),
+                   (2**32 - 300, 2**32 - 1), (77, 77)):
+        x = W.uint_range(lo + 1, n, lo, hi)
+        # a few keys far outside the window: rarely sampled, they land in
+        # the guard buckets (and are the first and last order statistics)
+        x[[5, 999, 123456]] = [0, 2**32 - 1, max(lo - 1000, 0)]
+        if dtype == np.float32:  # the same window of float keys
+            x = (x & np.uint32(0x7F7FFFFF)).view(np.float32)
+        k = np.concatenate([W.k_uniform(lo % 97, n, 100, dedup=False), [1, 2, 3, n - 1, n]]).astype(np.uint32)
+        d = pkg.DeviceArray.from_host(x)
+        got = pkg.multiselect(d, k, kind="radix")
+        assert np.array_equal(bits(got), bits(O.sort_choose(x, k))), (lo, hi)
+        d.free()
