@@ -149,9 +149,11 @@ def cpu_baseline(pkg, x: np.ndarray, k: np.ndarray, gpu_out: np.ndarray):
     }
 
 
-def timed_selects(pkg, L, d, n, dk, dout, steps, kind="bucket", K=0, ctx=None):
+def timed_selects(pkg, L, d, n, dk, dout, steps, kind="bucket", K=0, ctx=None, stages=None):
     """Mean wall time (ms) of `steps` stream-ordered selects between two
-    synchronisations, after three warm-up calls."""
+    synchronisations, after three warm-up calls.  With a dict `stages`, a
+    short untimed pass afterwards fills it with per-stage microseconds (HIP
+    events between the kernels, which add a little time of their own)."""
     lib = L.lib()
     ctx = ctx or pkg.Context(0)
     code = pkg.api.dtype_code(d.dtype)
@@ -168,7 +170,20 @@ def timed_selects(pkg, L, d, n, dk, dout, steps, kind="bucket", K=0, ctx=None):
     for _ in range(steps):
         one()
     L.check(lib.ggks_sync())
-    return (time.perf_counter() - t0) * 1e3 / steps
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    if stages is not None:
+        per, nprof = L.NSTAGES + 1, 3
+        L.check(lib.ggks_ctx_profile_marks(ctx.handle, (1 << per) - 1))
+        L.check(lib.ggks_ctx_profile_arm(ctx.handle, nprof))
+        for _ in range(nprof):
+            one()
+        L.check(lib.ggks_sync())
+        buf = np.zeros(nprof * per, np.float32)
+        L.check(lib.ggks_ctx_profile_read(ctx.handle, buf.ctypes.data, nprof))
+        L.check(lib.ggks_ctx_profile_arm(ctx.handle, 0))
+        buf = buf.reshape(nprof, per)
+        stages.update({nm: round(float(buf[:, i].mean()) * 1e3, 2) for i, nm in enumerate(L.STAGES)})
+    return ms
 
 
 def topk_radix_legs(pkg, L, d, x, k, bucket_out, steps):
@@ -188,12 +203,14 @@ def topk_radix_legs(pkg, L, d, x, k, bucket_out, steps):
             "bit_exact": bool(np.array_equal(got.view(u), want.view(u)))}
     dk = pkg.DeviceArray.from_host(k)
     dout = pkg.DeviceArray(k.size, x.dtype)
-    ms = timed_selects(pkg, L, d, n, dk, dout, steps, kind="radix")
+    st = {}
+    ms = timed_selects(pkg, L, d, n, dk, dout, steps, kind="radix", stages=st)
     got = dout.to_host()
     radix = {"workload": f"radix multiselect, n=2^{int(np.log2(n))} {x.dtype} N(0,1), {k.size} uniform k",
              "ms_per_step": round(ms, 4), "gelem_per_s": round(n / ms / 1e6, 2),
              "hbm_frac_call": round(2 * n * esz / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-             "identical_to_bucket": bool(np.array_equal(got.view(u), bucket_out.view(u)))}
+             "identical_to_bucket": bool(np.array_equal(got.view(u), bucket_out.view(u))),
+             "stage_us": st}
     return topk, radix
 
 
@@ -221,10 +238,12 @@ def adversarial_leg(pkg, L, steps):
         rec = {"input": name, "n": n, "num_k": int(k.size)}
         res = {}
         for kind in ("bucket", "radix"):
-            ms = timed_selects(pkg, L, d, n, dk, dout, steps, kind=kind, ctx=ctx)
+            st = {}
+            ms = timed_selects(pkg, L, d, n, dk, dout, steps, kind=kind, ctx=ctx, stages=st)
             res[kind] = dout.to_host()
             rec[kind] = {"ms_per_step": round(ms, 4), "gelem_per_s": round(n / ms / 1e6, 2),
-                         "hbm_frac_call": round(2 * n * 4 / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+                         "hbm_frac_call": round(2 * n * 4 / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                         "stage_us": st}
         ok = bool(np.array_equal(res["bucket"].view(np.uint32), res["radix"].view(np.uint32)))
         ok &= bool(res["bucket"][-2] == x.max() and res["bucket"][-1] == x.min())
         if dt == np.uint32:  # the k-th largest from the value histogram

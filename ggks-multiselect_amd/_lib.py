@@ -61,7 +61,7 @@ class Opts(ctypes.Structure):
     _fields_ = [("blocks", ctypes.c_int32), ("threads", ctypes.c_int32),
                 ("buckets_log2", ctypes.c_int32), ("destructive", ctypes.c_int32),
                 ("seed", ctypes.c_uint64), ("radix_bits", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 5)]
+                ("level2", ctypes.c_int32), ("reserved", ctypes.c_int32 * 4)]
 
 
 class Segment(ctypes.Structure):

@@ -116,10 +116,10 @@ class Context:
             pass
 
 
-def _opts(blocks=0, threads=0, buckets_log2=0, seed=0, radix_bits=0):
+def _opts(blocks=0, threads=0, buckets_log2=0, seed=0, radix_bits=0, level2=0):
     o = Opts()
     o.blocks, o.threads, o.buckets_log2 = int(blocks), int(threads), int(buckets_log2)
-    o.seed, o.radix_bits = int(seed), int(radix_bits)
+    o.seed, o.radix_bits, o.level2 = int(seed), int(radix_bits), int(level2)
     return o
 
 

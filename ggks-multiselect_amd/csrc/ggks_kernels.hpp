@@ -564,7 +564,11 @@ __global__ __launch_bounds__(kThreads) void k_lut(const SegDesc* __restrict__ se
                                                   const KT* __restrict__ samples,
                                                   const KT* __restrict__ minmax,
                                                   SegState* __restrict__ states,
-                                                  uint32_t* __restrict__ lut_pool) {
+                                                  uint32_t* __restrict__ lut_pool,
+                                                  uint32_t* __restrict__ l2count) {
+  // the second level's counters (heavy children, chunks) start at zero in
+  // every run; k_findk, which counts them, runs after this kernel
+  if (l2count && blockIdx.x == 0 && threadIdx.x < 2) l2count[threadIdx.x] = 0;
   __shared__ uint32_t cnt[1u << kMaxPrefixBits];
   __shared__ uint32_t wsum[17];
   __shared__ KT red_lo[16], red_hi[16];
@@ -645,6 +649,17 @@ __global__ __launch_bounds__(kThreads) void k_lut(const SegDesc* __restrict__ se
   } else {
     s = bl > (int)pb ? (uint32_t)(bl - (int)pb) : 0u;
     P = (uint32_t)(((uint64_t)R >> s) + 1);  // <= 2^pb
+    // guard prefixes below kmin and above kmax's prefix when the LUT has
+    // room (narrow key ranges): keys outside the sampled range then get
+    // buckets of their own instead of widening the extreme sampled keys'
+    // buckets to the ends of the key space (so those stay single-key when
+    // s = 0).  Each guard is added only where keys can lie beyond the range.
+    if (P + 2 <= (1u << pb)) {
+      const uint32_t glo_s = kmin >= (KT(1) << s) ? 1u : 0u;
+      kb = (KT)(kmin - ((KT)glo_s << s));
+      const bool ghi_s = (KT)(KeyBits<KT>::MAX - kb) >> s >= (KT)(P + glo_s);
+      P += glo_s + (ghi_s ? 1u : 0u);
+    }
   }
   const uint32_t Bt = sd.bmax - (hvon ? 2u : 0u);  // bucket budget of L
   const uint32_t cap = s >= 16 ? 65535u : min(65535u, 1u << s);
@@ -666,7 +681,7 @@ __global__ __launch_bounds__(kThreads) void k_lut(const SegDesc* __restrict__ se
     __syncthreads();
     // the whole sample in one round of loads (64 per thread for 4-byte keys)
     strided_for<64 * 4 / sizeof(KT)>(samples + sd.sample_off, sd.nsample, [&](uint32_t, KT k) {
-      atomicAdd(&cnt[(uint32_t)((k - kmin) >> s)], 1u);
+      atomicAdd(&cnt[(uint32_t)((KT)(k - kb) >> s)], 1u);
     });
     __syncthreads();
     // prefixes holding no sample: a run of them shares one bucket
@@ -770,6 +785,11 @@ __global__ __launch_bounds__(kThreads) void k_count(const KT* __restrict__ src,
   uint64_t r0 = 0;
   // full rounds, software pipelined: the next round's loads are in flight
   // while a round is bucketed and counted
+  // With a heavy value its bucket takes up to most keys: those are counted
+  // per wavefront (ballot and popcount, one LDS atomic per wavefront at the
+  // end) instead of one same-address LDS atomic per key.
+  const uint32_t hbk = st.hb;
+  uint32_t hvc = 0;  // wave-uniform
   auto full_rounds = [&](auto left, auto hvt) {
     if (r0 + per_round > nvec) return;  // uniform
     uint4 v[R];
@@ -785,8 +805,17 @@ __global__ __launch_bounds__(kThreads) void k_count(const KT* __restrict__ src,
       }
       uint32_t b[N];
       bucket_batch1<KT, C, N, decltype(left)::value, decltype(hvt)::value>(bk, lut, k, b);
+      if (decltype(hvt)::value) {
 #pragma unroll
-      for (int i = 0; i < N; ++i) atomicAdd(&hist[b[i]], 1u);
+        for (int i = 0; i < N; ++i) {
+          const bool h = b[i] == hbk;
+          hvc += (uint32_t)__popcll(__ballot(h));
+          if (!h) atomicAdd(&hist[b[i]], 1u);
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < N; ++i) atomicAdd(&hist[b[i]], 1u);
+      }
     }
   };
   // uniform: the sub-field shift direction and the heavy-value term
@@ -806,6 +835,7 @@ __global__ __launch_bounds__(kThreads) void k_count(const KT* __restrict__ src,
 #pragma unroll
     for (uint32_t j = 0; j < VEC; ++j) atomicAdd(&hist[b[j]], 1u);
   }
+  if (st.hvon && lane_id() == 0 && hvc) atomicAdd(&hist[hbk], hvc);
   __syncthreads();
   uint32_t* row = rows + sd.row_base + (uint64_t)ch.idx * bmax;
   for (uint32_t i = threadIdx.x; i < st.B; i += kThreads) row[i] = hist[i];
@@ -927,7 +957,10 @@ __device__ __forceinline__ void findk_segment(const SegDesc* __restrict__ segs,
                                               Query* __restrict__ cq,
                                               uint32_t* __restrict__ bits_pool,
                                               uint32_t* __restrict__ wbase_pool,
-                                              KT* __restrict__ out) {
+                                              KT* __restrict__ out,
+                                              uint32_t* __restrict__ l2count,
+                                              Heavy* __restrict__ heavy,
+                                              Chunk2* __restrict__ chunks2, uint32_t hmax) {
   extern __shared__ __align__(16) uint32_t lds[];
   const SegDesc sd = segs[blockIdx.x];
   // the first k of this thread, read now so its latency hides under the
@@ -1075,12 +1108,54 @@ __device__ __forceinline__ void findk_segment(const SegDesc* __restrict__ segs,
   }
   __syncthreads();
   const uint32_t nq = block_scan_array(hist, U, wsum);  // hist[u] = first slot
+  const bool l2 = (sd.flags & SEG_LEVEL2) && l2count;
   for (uint32_t u = tid; u < U; u += kThreads) {
     uint32_t first = hist[u];
     uint32_t next = (u + 1 < U) ? hist[u + 1] : nq;
-    children[sd.q_off + u].q_off = sd.q_off + first;
-    children[sd.q_off + u].q_cnt = next - first;
+    Child* cp = &children[sd.q_off + u];  // this thread wrote the record above
+    cp->q_off = sd.q_off + first;
+    cp->q_cnt = next - first;
+    const uint64_t clen = cp->len;
+    if (l2 && clen > kHeavyKeys) {
+      // a heavy child goes to the second level: a heavy-child record and its
+      // chunks; its slot finishes nothing until k_count2 writes a
+      // second-level child there (the capacity is the planner's exact
+      // bound, so a refused registration leaves an ordinary child)
+      const uint32_t h = atomicAdd(&l2count[0], 1u);
+      if (h < hmax) {
+        const uint32_t nch = (uint32_t)((clen + kChunk2 - 1) / kChunk2);
+        const uint32_t cb = atomicAdd(&l2count[1], nch);
+        Heavy H;
+        H.start = cp->start;
+        H.len = clen;
+        H.lo = cp->lo;
+        H.hi = cp->hi;
+        H.seg = blockIdx.x;
+        H.slot = sd.q_off + u;
+        H.q_off = sd.q_off + first;
+        H.q_cnt = next - first;
+        H.chunk_base = cb;
+        H.nchunks = nch;
+        const int rb = bitlen<KT>((KT)(cp->hi - cp->lo));
+        H.shift = rb > (int)kBits2 ? (uint32_t)(rb - (int)kBits2) : 0u;
+        H.done = 0;
+        H.nact = 0;
+        H.slot_base = 0;
+        H.pad[0] = H.pad[1] = 0;
+        heavy[h] = H;
+        for (uint32_t i = 0; i < nch; ++i) {
+          Chunk2 c2;
+          c2.h = h;
+          c2.idx = i;
+          c2.start = (uint64_t)i * kChunk2;
+          c2.len = min(kChunk2, clen - c2.start);
+          chunks2[cb + i] = c2;
+        }
+        cp->q_cnt = 0;
+      }
+    }
   }
+  if (l2 && tid == 0) states[blockIdx.x].nfree = U;  // first free child slot
   __syncthreads();
   for (uint32_t i = tid; i < sd.q_cnt; i += kThreads) {
     uint32_t u = qb[sd.q_off + i];
@@ -1105,9 +1180,12 @@ __device__ __forceinline__ void findk_segment(const SegDesc* __restrict__ segs,
       const uint32_t *__restrict__ gsums, const Query *__restrict__ queries,                   \
       const uint32_t *__restrict__ kraw, uint32_t *__restrict__ status, uint32_t *__restrict__ qb, \
       uint32_t *__restrict__ qr, Child *__restrict__ children, Query *__restrict__ cq,         \
-      uint32_t *__restrict__ bits_pool, uint32_t *__restrict__ wbase_pool, KT *__restrict__ out
-#define GGKS_FINDK_ARGS \
-  segs, states, lut_pool, totals, gsums, queries, kraw, status, qb, qr, children, cq, bits_pool, wbase_pool, out
+      uint32_t *__restrict__ bits_pool, uint32_t *__restrict__ wbase_pool, KT *__restrict__ out, \
+      uint32_t *__restrict__ l2count, Heavy *__restrict__ heavy, Chunk2 *__restrict__ chunks2,  \
+      uint32_t hmax
+#define GGKS_FINDK_ARGS                                                                        \
+  segs, states, lut_pool, totals, gsums, queries, kraw, status, qb, qr, children, cq, bits_pool, \
+      wbase_pool, out, l2count, heavy, chunks2, hmax
 constexpr uint32_t kFindkSmallBuckets = 4 * kThreads;  // one 16-byte word per thread
 template <typename KT, int C>
 __global__ __launch_bounds__(kThreads) void k_findk(GGKS_FINDK_PARAMS) {
@@ -1368,6 +1446,233 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) v
 }
 #undef GGKS_COMPACT_PARAMS
 
+// ------------------------------------------------------------ second level
+// Work-groups of both kernels walk the chunk table (count in l2count[1],
+// written by k_findk) with a grid stride, so one launch of a fixed grid
+// serves any number of heavy children.
+
+// Resolve heavy child h from its chunk rows (run by the work-group that
+// counted its last chunk): bucket totals and their prefix, each rank's
+// bucket (a single-key bucket answers the rank), the second-level children
+// (slots: the heavy child's own, then free slots of its segment), their
+// ranks regrouped into cq2, each chunk's start inside every active bucket,
+// and the active bitmap with its child bases for k_compact2.
+template <typename KT, int C>
+__device__ void resolve_heavy(uint32_t h, Heavy* __restrict__ heavy, uint32_t* __restrict__ rows2,
+                              const Query* __restrict__ cq, Query* __restrict__ cq2,
+                              uint32_t* __restrict__ qb, uint32_t* __restrict__ qr,
+                              Child* __restrict__ children, SegState* __restrict__ states,
+                              const SegDesc* __restrict__ segs, uint32_t* __restrict__ bits2,
+                              uint32_t* __restrict__ wb2, KT* __restrict__ out,
+                              uint32_t* hist, uint32_t* cnt, uint32_t* bits, uint32_t* wb,
+                              uint32_t* wsum, uint32_t* misc) {
+  constexpr uint32_t NW = kBuckets2 / 32;
+  const uint32_t tid = threadIdx.x;
+  const Heavy H = heavy[h];
+  const KT lo = (KT)H.lo, hi = (KT)H.hi;
+  const uint32_t sh = H.shift;
+  for (uint32_t b = tid; b < kBuckets2; b += kThreads) {
+    uint32_t t = 0;
+    for (uint32_t c = 0; c < H.nchunks; ++c) t += rows2[(uint64_t)(H.chunk_base + c) * kBuckets2 + b];
+    hist[b] = t;
+  }
+  for (uint32_t i = tid; i < NW; i += kThreads) bits[i] = 0;
+  __syncthreads();
+  const uint32_t total = block_scan_array(hist, kBuckets2, wsum);  // hist = prefix (synchronises)
+  const uint32_t NONE = 0xFFFFFFFFu;
+  auto count_of = [&](uint32_t b) { return (b + 1 < kBuckets2 ? hist[b + 1] : total) - hist[b]; };
+  auto range_of = [&](uint32_t b, KT& a, KT& z) {
+    a = (KT)(lo + ((KT)b << sh));
+    const KT w = (KT)((KT(1) << sh) - 1);
+    z = (KT)(hi - a) > w ? (KT)(a + w) : hi;
+  };
+  for (uint32_t i = tid; i < H.q_cnt; i += kThreads) {
+    const Query q = cq[H.q_off + i];
+    const uint32_t r1 = q.rank - 1;
+    uint32_t a = 0, z = kBuckets2 - 1;  // largest b with hist[b] <= r1
+    while (a < z) {
+      const uint32_t mid = (a + z + 1) >> 1;
+      if (hist[mid] <= r1) a = mid; else z = mid - 1;
+    }
+    KT klo, khi;
+    range_of(a, klo, khi);
+    if (klo == khi) {
+      out[q.out] = from_key<C>(klo);
+      qb[H.q_off + i] = NONE;
+    } else {
+      atomicOr(&bits[a >> 5], 1u << (a & 31));
+      qb[H.q_off + i] = a;
+      qr[H.q_off + i] = q.rank - hist[a];
+    }
+  }
+  __syncthreads();
+  uint32_t U2;
+  const uint32_t ub = block_excl_scan(tid < NW ? (uint32_t)__popc(bits[tid]) : 0u, wsum, U2);
+  if (tid < NW) {
+    wb[tid] = ub;
+    bits2[(uint64_t)h * NW + tid] = bits[tid];
+    wb2[(uint64_t)h * NW + tid] = ub;
+  }
+  if (tid == 0) {
+    misc[0] = U2 > 1 ? atomicAdd(&states[H.seg].nfree, U2 - 1) : 0u;
+    heavy[h].nact = U2;
+  }
+  __syncthreads();
+  if (U2 == 0) return;
+  const uint32_t sbase = misc[0];
+  if (tid == 0) heavy[h].slot_base = sbase;
+  const uint32_t sq = segs[H.seg].q_off;
+  auto slot_of = [&](uint32_t u) { return u == 0 ? H.slot : sq + sbase + u - 1; };
+  // children: one thread per child finds its bucket from the bitmap
+  for (uint32_t u = tid; u < U2; u += kThreads) {
+    uint32_t a = 0, z = NW - 1;  // largest word with wb <= u
+    while (a < z) {
+      const uint32_t mid = (a + z + 1) >> 1;
+      if (wb[mid] <= u) a = mid; else z = mid - 1;
+    }
+    uint32_t mw = bits[a];
+    for (uint32_t r = u - wb[a]; r; --r) mw &= mw - 1;
+    const uint32_t b = a * 32 + __ffs(mw) - 1;
+    cnt[u] = 0;
+    Child c;
+    c.start = H.start + hist[b];  // the second reduced buffer mirrors the first
+    c.len = count_of(b);
+    KT klo, khi;
+    range_of(b, klo, khi);
+    c.lo = klo;
+    c.hi = khi;
+    c.q_off = 0;
+    c.q_cnt = 0;
+    c.src = 2;
+    c.need_range = 0;
+    c.bucket = b;
+    c.seg = H.seg;
+    children[slot_of(u)] = c;
+    // each chunk's start inside this bucket's slice (relative to H.start)
+    uint32_t run = hist[b];
+    for (uint32_t ch = 0; ch < H.nchunks; ++ch) {
+      uint32_t* e = &rows2[(uint64_t)(H.chunk_base + ch) * kBuckets2 + b];
+      const uint32_t v = *e;
+      *e = run;
+      run += v;
+    }
+  }
+  __syncthreads();
+  // ranks grouped by child into cq2 (same index range as the heavy child's)
+  for (uint32_t i = tid; i < H.q_cnt; i += kThreads) {
+    const uint32_t b = qb[H.q_off + i];
+    if (b == NONE) continue;
+    const uint32_t u = wb[b >> 5] + __popc(bits[b >> 5] & ((1u << (b & 31)) - 1u));
+    atomicAdd(&cnt[u], 1u);
+    qb[H.q_off + i] = u;
+  }
+  __syncthreads();
+  const uint32_t nq2 = block_scan_array(cnt, U2, wsum);  // cnt[u] = first position
+  for (uint32_t u = tid; u < U2; u += kThreads) {
+    const uint32_t first = cnt[u], next = u + 1 < U2 ? cnt[u + 1] : nq2;
+    Child* cp = &children[slot_of(u)];  // written by this thread above
+    cp->q_off = H.q_off + first;
+    cp->q_cnt = next - first;
+  }
+  __syncthreads();
+  for (uint32_t i = tid; i < H.q_cnt; i += kThreads) {
+    const uint32_t u = qb[H.q_off + i];
+    if (u == NONE) continue;
+    const uint32_t pos = atomicAdd(&cnt[u], 1u);
+    Query q;
+    q.rank = qr[H.q_off + i];
+    q.out = cq[H.q_off + i].out;
+    cq2[H.q_off + pos] = q;
+  }
+}
+
+// Second-level count: per chunk of a heavy child, a histogram of its keys
+// (in the first reduced buffer) over the child's digit (key - lo) >> shift,
+// written as the chunk's row; the work-group that counts a child's last
+// chunk (release fence, counter, acquire fence) resolves the child.
+template <typename KT, int C>
+__global__ __launch_bounds__(kThreads) void k_count2(
+    const KT* __restrict__ red, Heavy* __restrict__ heavy, const Chunk2* __restrict__ chunks2,
+    const uint32_t* __restrict__ l2count, uint32_t cmax, uint32_t* __restrict__ rows2,
+    const Query* __restrict__ cq, Query* __restrict__ cq2, uint32_t* __restrict__ qb,
+    uint32_t* __restrict__ qr, Child* __restrict__ children, SegState* __restrict__ states,
+    const SegDesc* __restrict__ segs, uint32_t* __restrict__ bits2, uint32_t* __restrict__ wb2,
+    KT* __restrict__ out) {
+  __shared__ uint32_t hist[kBuckets2];
+  __shared__ uint32_t cnt[kBuckets2];
+  __shared__ uint32_t bits[kBuckets2 / 32], wb[kBuckets2 / 32];
+  __shared__ uint32_t wsum[17], misc[2];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t nchunks = min(l2count[1], cmax);
+  for (uint32_t w = blockIdx.x; w < nchunks; w += gridDim.x) {  // uniform
+    const Chunk2 ck = chunks2[w];
+    const KT lo = (KT)heavy[ck.h].lo;
+    const uint32_t sh = heavy[ck.h].shift;
+    const uint64_t hstart = heavy[ck.h].start;
+    const uint32_t hn = heavy[ck.h].nchunks;
+    for (uint32_t b = tid; b < kBuckets2; b += kThreads) hist[b] = 0;
+    __syncthreads();
+    for_each_elem(red + hstart + ck.start, ck.len, [&](KT raw) {
+      atomicAdd(&hist[(uint32_t)((KT)(to_key<C>(raw) - lo) >> sh)], 1u);
+    });
+    __syncthreads();
+    uint32_t* row = rows2 + (uint64_t)w * kBuckets2;
+    for (uint32_t b = tid; b < kBuckets2; b += kThreads) row[b] = hist[b];
+    __threadfence();  // release: the row is visible device-wide before the count
+    __syncthreads();
+    if (tid == 0) misc[1] = atomicAdd(&heavy[ck.h].done, 1u) == hn - 1 ? 1u : 0u;
+    __syncthreads();
+    if (misc[1]) {  // uniform: every chunk of ck.h is counted
+      __threadfence();  // acquire: the other work-groups' rows
+      resolve_heavy<KT, C>(ck.h, heavy, rows2, cq, cq2, qb, qr, children, states, segs, bits2, wb2,
+                           out, hist, cnt, bits, wb, wsum, misc);
+    }
+    __syncthreads();
+  }
+}
+
+// Second-level compaction: the keys of each heavy child's active buckets,
+// from the first reduced buffer into the second at the same offsets, each
+// chunk at its own start inside every bucket (LDS cursors).
+template <typename KT, int C>
+__global__ __launch_bounds__(kThreads) void k_compact2(
+    const KT* __restrict__ red, const Heavy* __restrict__ heavy,
+    const Chunk2* __restrict__ chunks2, const uint32_t* __restrict__ l2count, uint32_t cmax,
+    const uint32_t* __restrict__ rows2, const uint32_t* __restrict__ bits2,
+    const uint32_t* __restrict__ wb2, KT* __restrict__ red2) {
+  constexpr uint32_t NW = kBuckets2 / 32;
+  __shared__ uint32_t cur[kBuckets2];
+  __shared__ uint32_t bits[NW], wb[NW];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t nchunks = min(l2count[1], cmax);
+  for (uint32_t w = blockIdx.x; w < nchunks; w += gridDim.x) {  // uniform
+    const Chunk2 ck = chunks2[w];
+    if (heavy[ck.h].nact == 0) continue;  // uniform
+    const KT lo = (KT)heavy[ck.h].lo;
+    const uint32_t sh = heavy[ck.h].shift;
+    const uint64_t hstart = heavy[ck.h].start;
+    if (tid < NW) {
+      const uint32_t m = bits2[(uint64_t)ck.h * NW + tid];
+      uint32_t u = wb2[(uint64_t)ck.h * NW + tid];
+      bits[tid] = m;
+      wb[tid] = u;
+      const uint32_t* row = rows2 + (uint64_t)w * kBuckets2 + tid * 32;
+      for (uint32_t mm = m; mm; mm &= mm - 1) cur[u++] = row[__ffs(mm) - 1];
+    }
+    __syncthreads();
+    KT* dst = red2 + hstart;
+    for_each_elem(red + hstart + ck.start, ck.len, [&](KT raw) {
+      const uint32_t b = (uint32_t)((KT)(to_key<C>(raw) - lo) >> sh);
+      const uint32_t m = bits[b >> 5], bit = b & 31;
+      if ((m >> bit) & 1u) {
+        const uint32_t u = wb[b >> 5] + __popc(m & ((1u << bit) - 1u));
+        dst[atomicAdd(&cur[u], 1u)] = raw;
+      }
+    });
+    __syncthreads();
+  }
+}
+
 // Plain streaming copy of nv 16-byte words (grid-stride).
 __global__ __launch_bounds__(kThreads) void k_copy16(const uint4* __restrict__ src,
                                                      uint4* __restrict__ dst, uint64_t nv) {
@@ -1539,9 +1844,11 @@ __device__ __forceinline__ void wave_pick256(const uint32_t* h, uint32_t lane, u
 template <typename KT, int C, int R>
 __device__ __forceinline__ void finish_small_body(const KT* __restrict__ input,
                                                   const KT* __restrict__ red,
+                                                  const KT* __restrict__ red2,
                                                   const Child* __restrict__ children,
                                                   uint32_t nchildren,
-                                                  const Query* __restrict__ cq,
+                                                  const Query* __restrict__ cq0,
+                                                  const Query* __restrict__ cq2,
                                                   KT* __restrict__ out) {
   constexpr uint32_t W = KeyBits<KT>::W;
   constexpr uint32_t CAP = kFCollectWords * 4 / sizeof(KT);  // gathered keys
@@ -1572,7 +1879,10 @@ __device__ __forceinline__ void finish_small_body(const KT* __restrict__ input,
     const uint32_t nl = list[kFThreads];
     for (uint32_t li = 0; li < nl; ++li) {
       const Child c = children[list[li]];
-      const KT* srcp = (c.src ? red : input) + c.start;
+      // source: the input, the first or the second reduced buffer; a
+      // second-level child's ranks are in cq2
+      const KT* srcp = (c.src == 0 ? input : (c.src == 1 ? red : red2)) + c.start;
+      const Query* __restrict__ cq = c.src == 2 ? cq2 : cq0;
       if (sort_child(c)) {  // uniform
         // many ranks: the converted keys sorted ascending in LDS (bitonic,
         // one compare-exchange pair per thread step); rank r is the r-th key
@@ -1761,17 +2071,18 @@ __device__ __forceinline__ void finish_small_body(const KT* __restrict__ input,
 }
 
 #define GGKS_FINISH_PARAMS                                                                    \
-  const KT *__restrict__ input, const KT *__restrict__ red, const Child *__restrict__ children, \
-      uint32_t nchildren, const Query *__restrict__ cq, KT *__restrict__ out
+  const KT *__restrict__ input, const KT *__restrict__ red, const KT *__restrict__ red2,      \
+      const Child *__restrict__ children, uint32_t nchildren, const Query *__restrict__ cq,   \
+      const Query *__restrict__ cq2, KT *__restrict__ out
 template <typename KT, int C>
 __global__ __launch_bounds__(kFThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_finish_small(
     GGKS_FINISH_PARAMS) {
-  finish_small_body<KT, C, kFRegWords<KT>>(input, red, children, nchildren, cq, out);
+  finish_small_body<KT, C, kFRegWords<KT>>(input, red, red2, children, nchildren, cq, cq2, out);
 }
 template <typename KT, int C>
 __global__ __launch_bounds__(kFThreads) __attribute__((amdgpu_waves_per_eu(6))) void k_finish_small_lite(
     GGKS_FINISH_PARAMS) {
-  finish_small_body<KT, C, kFLiteWords>(input, red, children, nchildren, cq, out);
+  finish_small_body<KT, C, kFLiteWords>(input, red, red2, children, nchildren, cq, cq2, out);
 }
 #undef GGKS_FINISH_PARAMS
 
@@ -1790,9 +2101,11 @@ constexpr uint32_t kFinishLdsTotal = kFinishOffList + (kThreads + 4) * 4;
 template <typename KT, int C>
 __global__ __launch_bounds__(kThreads) void k_finish(const KT* __restrict__ input,
                                                      const KT* __restrict__ red,
+                                                     const KT* __restrict__ red2,
                                                      const Child* __restrict__ children,
                                                      uint32_t nchildren,
-                                                     const Query* __restrict__ cq,
+                                                     const Query* __restrict__ cq0,
+                                                     const Query* __restrict__ cq2,
                                                      KT* __restrict__ out) {
   constexpr uint32_t W = KeyBits<KT>::W;
   constexpr uint32_t CAP = kFinishLdsBytes / sizeof(KT);
@@ -1829,7 +2142,8 @@ __global__ __launch_bounds__(kThreads) void k_finish(const KT* __restrict__ inpu
     const uint32_t nl = nlist;
     for (uint32_t li = 0; li < nl; ++li) {
       const Child c = children[list[li]];
-      const KT* srcp = (c.src ? red : input) + c.start;
+      const KT* srcp = (c.src == 0 ? input : (c.src == 1 ? red : red2)) + c.start;
+      const Query* __restrict__ cq = c.src == 2 ? cq2 : cq0;
       const bool inl = c.len <= CAP;
       KT lo = (KT)c.lo, hi = (KT)c.hi;
       if (inl) {  // the child into LDS, eight loads per thread in flight

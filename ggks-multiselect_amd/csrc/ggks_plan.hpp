@@ -58,6 +58,11 @@ struct Plan {
   // longest direct child, and the largest mean bucket size of an engine
   // segment (a radix segment's whole length): picks the finisher variant
   uint64_t dmax_keys = 0, emax_keys = 0;
+  // second level: enabled, capacity (heavy children, chunks) and regions
+  bool level2 = false;
+  uint32_t hmax = 0, cmax = 0;
+  size_t off_hcount = 0, off_heavy = 0, off_chunks2 = 0, off_rows2 = 0, off_bits2 = 0, off_wb2 = 0;
+  size_t off_cq2 = 0, off_red2 = 0;
   uint64_t seed = 0;
   // arena offsets (bytes)
   size_t off_tables = 0, off_segs = 0, off_chunks = 0, off_items = 0, off_sblocks = 0, tables_end = 0;
@@ -90,6 +95,33 @@ struct Plan {
              direct_queries.size() * sizeof(Query));
   }
 };
+
+// Whether a segment's heavy children go to the second level (ggks_opts
+// level2: 0 = radix segments only, 1 = every segment, -1 = none), and the
+// most heavy children it can have (each holds a rank and more than
+// kHeavyKeys keys).
+inline bool seg_level2(const ggks_opts& o, uint32_t mode) {
+  return o.level2 > 0 || (o.level2 == 0 && mode == MODE_RADIX);
+}
+inline uint64_t seg_heavy_max(uint64_t len, uint32_t nq) {
+  return std::min<uint64_t>(nq, len / (kHeavyKeys + 1));
+}
+
+// Second-level regions at the end of an arena layout (tables and scratch
+// sized by the capacity: hmax heavy children, cmax chunks of kChunk2 keys,
+// red elements in the second reduced buffer, nq query slots).
+template <typename Take>
+inline void layout_level2(Plan& p, Take&& take, uint64_t red, uint64_t nq, int eb) {
+  if (!p.level2) return;
+  p.off_hcount = take(256);
+  p.off_heavy = take((size_t)p.hmax * sizeof(Heavy));
+  p.off_chunks2 = take((size_t)p.cmax * sizeof(Chunk2));
+  p.off_rows2 = take((size_t)p.cmax * kBuckets2 * 4);
+  p.off_bits2 = take((size_t)p.hmax * (kBuckets2 / 32) * 4);
+  p.off_wb2 = take((size_t)p.hmax * (kBuckets2 / 32) * 4);
+  p.off_cq2 = take((size_t)nq * sizeof(Query));
+  p.off_red2 = take((size_t)red * eb);
+}
 
 // The shape of a single-vector plan: its route, bucket function parameters,
 // chunking and sample size, computed from scalars alone (no tables).
@@ -164,6 +196,7 @@ inline SegDesc single_segment(const SingleShape& sh, int kind, uint64_t n, uint3
   s.radix_bits = sh.rb;
   s.qsrc = kind == GGKS_KIND_TOPK ? QSRC_TOPK : QSRC_K;
   s.out_off = 0;
+  s.flags = 0;
   return s;
 }
 
@@ -224,6 +257,15 @@ inline void plan_single(Plan& p, int kind, int dtype, uint64_t n, uint32_t nq,
       p.items.push_back({0u, (uint16_t)(i << sh.gl), (uint16_t)sh.gl});
     for (uint32_t y = 0; y < ns / kThreads; ++y) p.sblocks.push_back({0u, y});
   }
+  if (seg_level2(o, sh.mode)) {
+    const uint64_t hm = seg_heavy_max(n, nq);
+    if (hm > 0) {
+      p.level2 = true;
+      p.hmax = (uint32_t)hm;
+      p.cmax = (uint32_t)(n / kChunk2 + hm);
+    }
+  }
+  if (tables && p.level2) p.segs[0].flags |= SEG_LEVEL2;
   p.bmax_max = bmax;
   p.pbits_max = pb;
   p.nsegs = 1;
@@ -253,6 +295,7 @@ inline void plan_single(Plan& p, int kind, int dtype, uint64_t n, uint32_t nq,
   p.off_children = take((size_t)nq * sizeof(Child));
   p.off_cq = take((size_t)nq * sizeof(Query));
   p.off_red = take((size_t)n * eb);
+  layout_level2(p, take, n, nq, eb);
   p.total = off;
 }
 
@@ -335,6 +378,7 @@ inline bool plan_batch(Plan& p, int dtype, const std::vector<BatchSeg>& in, cons
   uint64_t red = 0, rowc = 0;
   // 64-bit running totals of the 32-bit offsets, checked before each use
   uint64_t qoff = 0, sam = 0, lutc = 0, totc = 0, bitc = 0, dq = 0;
+  uint64_t hmax = 0, cmax = 0;  // second-level capacity
   for (size_t i = 0; i < in.size(); ++i) {
     const BatchSeg& g = in[i];
     if (g.num_k == 0) continue;
@@ -418,6 +462,15 @@ inline bool plan_batch(Plan& p, int dtype, const std::vector<BatchSeg>& in, cons
     s.radix_bits = rb;
     s.qsrc = topk ? QSRC_TOPK : QSRC_K;
     s.out_off = (uint32_t)g.out_offset;
+    s.flags = 0;
+    if (seg_level2(o, mode)) {
+      const uint64_t hm = seg_heavy_max(n, g.num_k);
+      if (hm > 0) {
+        s.flags |= SEG_LEVEL2;
+        hmax += hm;
+        cmax += n / kChunk2 + hm;
+      }
+    }
     const uint32_t si = (uint32_t)p.segs.size();
     p.segs.push_back(s);
     for (uint64_t c = 0; c < G; ++c) {
@@ -444,6 +497,10 @@ inline bool plan_batch(Plan& p, int dtype, const std::vector<BatchSeg>& in, cons
     totc += bmax;
     bitc += (bmax + 31) / 32;
   }
+  if (hmax > kPlanTableLimit || cmax > kPlanGridLimit) return false;
+  p.level2 = hmax > 0;
+  p.hmax = (uint32_t)hmax;
+  p.cmax = (uint32_t)cmax;
   p.nq = (uint32_t)qoff;
   p.nsegs = (uint32_t)p.segs.size();
   p.nchunks = (uint32_t)p.chunks.size();
@@ -485,6 +542,7 @@ inline bool plan_batch(Plan& p, int dtype, const std::vector<BatchSeg>& in, cons
   p.off_children = take((size_t)qoff * sizeof(Child));
   p.off_cq = take((size_t)qoff * sizeof(Query));
   p.off_red = take((size_t)red * eb);
+  layout_level2(p, take, red, qoff, eb);
   p.total = off;
   return true;
 }

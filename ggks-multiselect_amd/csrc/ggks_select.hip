@@ -192,22 +192,27 @@ static void level_attrs(const Plan&) {
 // max_groups (256: one per CU) loops over the list without losing
 // concurrency.
 template <typename KT, int C>
-static void finish(hipStream_t s, const KT* d_src, const KT* red, const Child* kids, uint32_t n,
-                   const Query* cq, KT* d_out, uint32_t max_groups, bool lite) {
+static void finish(hipStream_t s, const KT* d_src, const KT* red, const KT* red2,
+                   const Child* kids, uint32_t n, const Query* cq, const Query* cq2, KT* d_out,
+                   uint32_t max_groups, bool lite) {
   bool launched = false;
   if constexpr (sizeof(KT) == 4) {
     if (lite) {
       hipLaunchKernelGGL((k_finish_small_lite<KT, C>), dim3(std::min<uint32_t>(n, 65536)),
-                         dim3(kFThreads), 0, s, d_src, red, kids, n, cq, d_out);
+                         dim3(kFThreads), 0, s, d_src, red, red2, kids, n, cq, cq2, d_out);
       launched = true;
     }
   }
   if (!launched)
     hipLaunchKernelGGL((k_finish_small<KT, C>), dim3(std::min<uint32_t>(n, 65536)),
-                       dim3(kFThreads), 0, s, d_src, red, kids, n, cq, d_out);
+                       dim3(kFThreads), 0, s, d_src, red, red2, kids, n, cq, cq2, d_out);
   hipLaunchKernelGGL((k_finish<KT, C>), dim3(std::min<uint32_t>(n, max_groups)), dim3(kThreads),
-                     kFinishLdsTotal, s, d_src, red, kids, n, cq, d_out);
+                     kFinishLdsTotal, s, d_src, red, red2, kids, n, cq, cq2, d_out);
 }
+
+// Grid of the second-level kernels (they walk the chunk table with a grid
+// stride): two 1024-thread work-groups per CU.
+constexpr uint32_t kLevel2Groups = 512;
 
 // The level pipeline over every engine segment of the plan, and the
 // finisher over the plan's direct children (whole small segments): after
@@ -232,7 +237,8 @@ static int enqueue_level(const Exec& x, const Plan& p, const KT* d_src, KT* d_ou
   auto finish_direct = [&](hipStream_t ds) {
     const Child* dk = reinterpret_cast<const Child*>(A + p.off_dchildren);
     const Query* dq = reinterpret_cast<const Query*>(A + p.off_dqueries);
-    finish<KT, C>(ds, d_src, (const KT*)nullptr, dk, ndirect, dq, d_out, 256, dlite);
+    finish<KT, C>(ds, d_src, (const KT*)nullptr, (const KT*)nullptr, dk, ndirect, dq,
+                  (const Query*)nullptr, d_out, 256, dlite);
   };
   const uint32_t nseg = p.nsegs;
   if (nseg > 0) {
@@ -255,6 +261,15 @@ static int enqueue_level(const Exec& x, const Plan& p, const KT* d_src, KT* d_ou
     Query* cq = reinterpret_cast<Query*>(A + p.off_cq);
     KT* red = reinterpret_cast<KT*>(A + p.off_red);
     const uint32_t nchunks = p.nchunks;
+    // second level (null pointers when the plan has none)
+    uint32_t* l2count = p.level2 ? reinterpret_cast<uint32_t*>(A + p.off_hcount) : nullptr;
+    Heavy* heavy = p.level2 ? reinterpret_cast<Heavy*>(A + p.off_heavy) : nullptr;
+    Chunk2* chunks2 = p.level2 ? reinterpret_cast<Chunk2*>(A + p.off_chunks2) : nullptr;
+    uint32_t* rows2 = p.level2 ? reinterpret_cast<uint32_t*>(A + p.off_rows2) : nullptr;
+    uint32_t* bits2 = p.level2 ? reinterpret_cast<uint32_t*>(A + p.off_bits2) : nullptr;
+    uint32_t* wb2 = p.level2 ? reinterpret_cast<uint32_t*>(A + p.off_wb2) : nullptr;
+    Query* cq2 = p.level2 ? reinterpret_cast<Query*>(A + p.off_cq2) : nullptr;
+    KT* red2 = p.level2 ? reinterpret_cast<KT*>(A + p.off_red2) : nullptr;
 
     stage(x, 1);
     if (p.nsblocks > 0) {
@@ -264,7 +279,7 @@ static int enqueue_level(const Exec& x, const Plan& p, const KT* d_src, KT* d_ou
     }
     stage(x, 2);
     hipLaunchKernelGGL((k_lut<KT>), dim3(nseg), dim3(kThreads), 0, s, segs, samples, minmax, states,
-                       lut);
+                       lut, l2count);
     stage(x, 3);
     if (beside) {  // nseg > 0
       HIP_TRY(hipEventRecord(x.fork, s));
@@ -284,13 +299,13 @@ static int enqueue_level(const Exec& x, const Plan& p, const KT* d_src, KT* d_ou
                          ((size_t)p.bmax_max + (1u << p.pbits_max) + 3 * ((p.bmax_max + 31) / 32) +
                           32) * 4,
                          s, segs, states, lut, tot, gsum, q, d_k, x.status, qb, qr, kids, cq, bits, wbase,
-                         d_out);
+                         d_out, l2count, heavy, chunks2, p.hmax);
     else
       hipLaunchKernelGGL((k_findk<KT, C>), dim3(nseg), dim3(kThreads),
                          ((size_t)p.bmax_max + (1u << p.pbits_max) + 3 * ((p.bmax_max + 31) / 32) +
                           32) * 4,
                          s, segs, states, lut, tot, gsum, q, d_k, x.status, qb, qr, kids, cq, bits, wbase,
-                         d_out);
+                         d_out, l2count, heavy, chunks2, p.hmax);
     stage(x, 6);
     if (compact_small(p.pbits_max, p.bmax_max))
       hipLaunchKernelGGL((k_compact_small<KT, C>), dim3(nchunks), dim3(kThreads),
@@ -301,7 +316,15 @@ static int enqueue_level(const Exec& x, const Plan& p, const KT* d_src, KT* d_ou
                          compact_lds_bytes<false>(), s, d_src, segs, states, chunks, lut, rows,
                          bits, wbase, kids, red);
     stage(x, 7);
-    finish<KT, C>(s, d_src, red, kids, p.nq, cq, d_out, 256, elite);
+    if (p.level2) {  // the heavy children's second level (part of the finish stage)
+      const uint32_t g2 = std::max<uint32_t>(1, std::min<uint32_t>(p.cmax, kLevel2Groups));
+      hipLaunchKernelGGL((k_count2<KT, C>), dim3(g2), dim3(kThreads), 0, s, red, heavy, chunks2,
+                         l2count, p.cmax, rows2, cq, cq2, qb, qr, kids, states, segs, bits2, wb2,
+                         d_out);
+      hipLaunchKernelGGL((k_compact2<KT, C>), dim3(g2), dim3(kThreads), 0, s, red, heavy, chunks2,
+                         l2count, p.cmax, rows2, bits2, wb2, red2);
+    }
+    finish<KT, C>(s, d_src, red, red2, kids, p.nq, cq, cq2, d_out, 256, elite);
   } else {
     for (int st = 1; st < GGKS_NSTAGES; ++st) stage(x, st);
   }
@@ -559,6 +582,7 @@ static int select_noalloc(int dtype, const void* d_vec, int64_t n, const uint32_
     a.dchild = single_direct_child((uint64_t)n, (uint32_t)num_k);
   } else {
     a.seg = single_segment(sh, GGKS_KIND_BUCKET, (uint64_t)n, (uint32_t)num_k);
+    if (p.level2) a.seg.flags |= SEG_LEVEL2;
     a.csz = sh.csz;
     a.G = (uint32_t)sh.G;
     a.nitems = sh.nitems;

@@ -62,7 +62,10 @@ struct SegDesc {          // planned on the host, read-only on the device
   uint32_t radix_bits;    // MODE_RADIX digit width
   uint32_t qsrc;          // QuerySource
   uint32_t out_off;       // first output slot (QSRC_K, QSRC_TOPK)
+  uint32_t flags;         // SEG_LEVEL2: its heavy children go to the second level
 };
+
+enum SegFlags : uint32_t { SEG_LEVEL2 = 1u };
 
 // The bucket function of a segment (BucketFn in ggks_kernels.hpp):
 //   low = min(max(key, kb) - kb, lmax)      keys outside the sampled range
@@ -90,7 +93,7 @@ struct SegState {         // written on the device by the LUT kernel
   uint32_t rsh, twid, lsh;// sub-bucket bit field and normalising shift
   uint32_t hvon;          // 1: the heavy-value bucket is in use
   uint32_t hb;            // F's bucket of hv (L(hv) + 1) when hvon
-  uint32_t pad;
+  uint32_t nfree;         // next free child slot (second-level children)
 };
 
 struct Chunk {            // one work-group of the streaming passes
@@ -131,6 +134,40 @@ struct Child {            // one bucket to finish, or one small whole segment
   uint32_t need_range;    // 1: lo/hi unknown, computed by the finisher
   uint32_t bucket;        // bucket of the parent segment it came from
   uint32_t seg;           // parent segment
+};
+
+// Second level.  A child of more than kHeavyKeys keys found by k_findk is
+// re-bucketed by the whole GPU instead of being finished by one work-group:
+// k_count2 histograms its keys in chunks of kChunk2 keys over kBuckets2
+// buckets (the exact digit (key - lo) >> shift of its key range), the last
+// chunk to finish resolves its ranks (single-key buckets are answered) and
+// lays out its second-level children, and k_compact2 copies their keys into
+// the second reduced buffer at the child's own offsets.  Its children take
+// the heavy child's slot and free slots of its segment's child table.
+constexpr uint64_t kHeavyKeys = 1ull << 16;
+constexpr uint64_t kChunk2 = 1ull << 18;
+constexpr uint32_t kBuckets2 = 4096;
+constexpr uint32_t kBits2 = 12;  // log2 kBuckets2
+
+struct Heavy {             // written by k_findk
+  uint64_t start, len;     // its keys in the first reduced buffer
+  uint64_t lo, hi;         // key range
+  uint32_t seg;            // first-level segment
+  uint32_t slot;           // its child slot
+  uint32_t q_off, q_cnt;   // its ranks (Query records of the child)
+  uint32_t chunk_base;     // first second-level chunk
+  uint32_t nchunks;
+  uint32_t shift;          // bucket = (key - lo) >> shift
+  uint32_t done;           // chunks counted (the last one resolves the ranks)
+  uint32_t nact;           // second-level children (k_compact2 skips 0)
+  uint32_t slot_base;      // child slots of children 1.. (child 0 takes `slot`)
+  uint32_t pad[2];
+};
+
+struct Chunk2 {
+  uint32_t h;              // heavy child
+  uint32_t idx;            // chunk index inside it
+  uint64_t start, len;     // inside the heavy child
 };
 
 // Per-context status bits written by kernels.

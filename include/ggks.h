@@ -61,7 +61,10 @@ typedef struct ggks_opts {
   int32_t destructive;  /* accepted; the input is never modified */
   uint64_t seed;        /* seed of the splitter sample; 0 = fixed default */
   int32_t radix_bits;   /* radix path: digit width per level (4..14); 0 = 14 */
-  int32_t reserved[5];
+  int32_t level2;       /* second level for children of more than 2^16 keys
+                           (re-bucketed by the whole GPU): 0 = radix plans
+                           only, 1 = every plan, -1 = never */
+  int32_t reserved[4];
 } ggks_opts;
 
 typedef struct ggks_ctx ggks_ctx;
