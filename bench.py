@@ -443,6 +443,7 @@ def main():
         sys.exit(2)
     pkg = load_package()
     from ggks_multiselect_amd import _lib as L  # noqa: E402
+    tuned = pkg.autotune.apply()  # the committed band tables (single-vector and batch)
     n = 1 << a.log2n
     t_gen = time.perf_counter()
     x = make_input(pkg, a.dtype, n)
@@ -546,6 +547,8 @@ def main():
                         f"{'N(0,1)' if a.dtype in ('f32', 'f64') else 'uniform'}, "
                         f"{k.size} uniform k (sorted, deduplicated)",
             "n": n, "num_k": int(k.size), "blocks": a.blocks, "buckets_log2": a.buckets_log2,
+            "tuning": {"table": "tuning_gfx950.json" if tuned else "built-in defaults",
+                       "buckets_log2_chunks": list(pkg.tuning_single_get(x.dtype, a.log2n))},
             "hbm_frac_call": round(call_frac, 4),
             "alg_bytes_per_step": 2 * n * esz,
             "dominant_kernel_us": round(dom_us, 2),

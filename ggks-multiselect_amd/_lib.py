@@ -29,12 +29,13 @@ EXPORTS = [
     "ggks_topk_i32",
     "ggks_bucket_multiselect_async", "ggks_radix_multiselect_async", "ggks_topk_async",
     "ggks_ctx_reserve", "ggks_ctx_check", "ggks_ctx_stream", "ggks_dev_alloc",
-    "ggks_bucket_multiselect_noalloc",
+    "ggks_bucket_multiselect_noalloc", "ggks_sort_choose",
     "ggks_dev_free", "ggks_h2d", "ggks_d2h", "ggks_d2d", "ggks_sync",
     "ggks_device_count", "ggks_set_device", "ggks_dev_fill", "ggks_ctx_profile_arm",
     "ggks_ctx_profile_read", "ggks_ctx_profile_marks", "ggks_batch_create", "ggks_batch_scratch_bytes", "ggks_batch_run",
     "ggks_batch_sync", "ggks_batch_stream", "ggks_batch_info", "ggks_batch_destroy",
     "ggks_tuning_set", "ggks_tuning_get", "ggks_tuning_reset", "ggks_generate",
+    "ggks_tuning_single_set", "ggks_tuning_single_get",
     "ggks_server_create", "ggks_server_submit", "ggks_server_poll", "ggks_server_drain",
     "ggks_server_run_trace", "ggks_server_stats_get", "ggks_server_destroy",
 ]
@@ -135,6 +136,7 @@ def lib() -> ctypes.CDLL:
         "ggks_topk_async": ([vp, i32, vp, i64, i64, vp, optp], i32),
         "ggks_ctx_reserve": ([vp, i32, i32, i64, i64], i32),
         "ggks_bucket_multiselect_noalloc": ([i32, vp, i64, vp, i64, vp, optp, vp], i32),
+        "ggks_sort_choose": ([vp, i32, vp, i64, vp, i64, vp], i32),
         "ggks_ctx_check": ([vp], i32),
         "ggks_ctx_stream": ([vp], vp),
         "ggks_dev_alloc": ([ctypes.POINTER(vp), sz], i32),
@@ -161,6 +163,8 @@ def lib() -> ctypes.CDLL:
         "ggks_tuning_get": ([i32, i32, ctypes.POINTER(i32), ctypes.POINTER(i32),
                              ctypes.POINTER(i32)], i32),
         "ggks_tuning_reset": ([], i32),
+        "ggks_tuning_single_set": ([i32, i32, i32, i32], i32),
+        "ggks_tuning_single_get": ([i32, i32, ctypes.POINTER(i32), ctypes.POINTER(i32)], i32),
         "ggks_generate": ([vp, i32, i32, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64,
                            i64, i64], i32),
         "ggks_server_create": ([ctypes.POINTER(vp), i32, vp, vp, sz, i32, i32], i32),

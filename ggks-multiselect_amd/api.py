@@ -305,6 +305,18 @@ def tuning_reset() -> None:
     check(lib().ggks_tuning_reset())
 
 
+def tuning_single_set(dtype, log2_len: int, buckets_log2: int, chunks: int) -> None:
+    check(lib().ggks_tuning_single_set(dtype_code(dtype), int(log2_len), int(buckets_log2),
+                                       int(chunks)))
+
+
+def tuning_single_get(dtype, log2_len: int) -> tuple:
+    b, c = ctypes.c_int(), ctypes.c_int()
+    check(lib().ggks_tuning_single_get(dtype_code(dtype), int(log2_len), ctypes.byref(b),
+                                       ctypes.byref(c)))
+    return b.value, c.value
+
+
 def generate(dtype, family: str, seed: int, n: int, start: int = 0, lo: int = 0, hi: int = 0,
              out: DeviceArray | None = None, offset: int = 0):
     """Fill a device array (new, or `out` from element `offset`) with n seeded
@@ -412,4 +424,5 @@ class Server:
 __all__ = ["DeviceArray", "DeviceView", "Context", "GGKSError", "multiselect", "topk",
            "multiselect_async", "scratch_bytes", "sync", "bucketMultiselectWrapper",
            "RadixSelectWrapper", "randomizedTopkSelectWrapper", "dtype_code", "Batch",
-           "segment_table", "tuning_set", "tuning_get", "tuning_reset", "generate", "Server"]
+           "segment_table", "tuning_set", "tuning_get", "tuning_reset", "generate", "Server",
+           "tuning_single_set", "tuning_single_get"]

@@ -1,4 +1,5 @@
-"""Autotuner of the batched path's band table.
+"""Autotuner of the band tables: the batched path's, and the single-vector
+one.
 
 For every (dtype, floor(log2 length)) band it times one representative batch
 (segments with lengths uniform in [2^b, 2^(b+1)), about `elems` elements in
@@ -12,7 +13,20 @@ taken.  The result is a JSON table that ``apply()`` loads into the library
 leaves its size cutoffs commented out (bucketMultiselect.cu:849-866); this
 measures them per band instead.
 
+Two more modes:
+- single: per (dtype, floor(log2 n)) the bucket budget and chunk count of
+  the single-vector entry points (one vector of 2^b keys, 1024 uniform k),
+  timed as back-to-back stream-ordered selects, fastest bit-identical
+  candidate kept (ggks_tuning_single_set);
+- mix: a coordinate descent of the batch bands inside the whole mixed batch
+  (the 4096-vector workload), because a band probed alone pays for its
+  direct segments' finishers, which in the mix run beside the pipeline; a
+  band's entry changes only when the whole batch gets faster by more than
+  the replay noise.
+
 Usage: python ggks-multiselect_amd/autotune.py [--dtypes f32,f64] [--bands 10-26]
+       python ggks-multiselect_amd/autotune.py --mode single [--bands 16-28]
+       python ggks-multiselect_amd/autotune.py --mode mix
 """
 from __future__ import annotations
 
@@ -122,6 +136,119 @@ def tune(dtypes=("f32", "f64"), bands=range(10, 27), elems: int = 1 << 26, reps:
     return table_out
 
 
+def single_candidates(band: int):
+    out = []
+    for bl in (11, 12, 13, 14, 15):
+        if bl > band - 4:
+            continue
+        for ch in (64, 128, 256, 384, 512):
+            if ch * 4096 > (1 << band):
+                continue
+            out.append((bl, ch))
+    return out
+
+
+def tune_single(dtypes=("f32", "f64", "u32", "i32"), bands=range(16, 29), reps: int = 10,
+                numk: int = 1024, seed: int = 91, log=print) -> dict:
+    """Per (dtype, band): the fastest bit-identical (bucket budget, chunks)
+    of the single-vector select on one resident vector of 2^band keys."""
+    pkg = _pkg()
+    lib = pkg.api.lib()
+    W = pkg.workloads
+    out = {}
+    for name in dtypes:
+        dt = DTYPES[name]
+        out[name] = {}
+        for band in bands:
+            n = 1 << band
+            if np.dtype(dt).itemsize == 8 and band > 27:
+                continue
+            fam = "normal" if np.dtype(dt).kind == "f" else "uniform"
+            hi = 0xFFFFFFFF if dt == np.uint32 else 2**31 - 1
+            lo_v = 0 if dt == np.uint32 else -2**31
+            d = pkg.generate(dt, fam, seed + band, n, lo=lo_v, hi=hi)
+            k = W.k_uniform(seed, n, min(numk, n // 4))
+            dk = pkg.DeviceArray.from_host(k)
+            dout = pkg.DeviceArray(k.size, dt)
+            ctx = pkg.Context(0)
+            best, ref = None, None
+            for bl, ch in single_candidates(band):
+                for _ in range(2):
+                    pkg.multiselect_async(d, dk, dout, ctx, buckets_log2=bl, blocks=ch)
+                pkg.api.check(lib.ggks_sync())
+                res = dout.to_host()
+                t0 = time.perf_counter()
+                for _ in range(reps):
+                    pkg.multiselect_async(d, dk, dout, ctx, buckets_log2=bl, blocks=ch)
+                pkg.api.check(lib.ggks_sync())
+                t = (time.perf_counter() - t0) / reps
+                if ref is None:
+                    ref = res
+                elif not np.array_equal(res.view(np.uint8), ref.view(np.uint8)):
+                    raise RuntimeError(f"{name} 2^{band} ({bl},{ch}): outputs differ")
+                if best is None or t < best[0]:
+                    best = (t, bl, ch)
+            if best is None:
+                continue
+            t, bl, ch = best
+            out[name][str(band)] = [bl, ch]
+            log(f"single {name} 2^{band}: buckets 2^{bl}, {ch} chunks, {t * 1e6:.1f} us, "
+                f"{n / t / 1e9:.1f} Gelem/s", flush=True)
+            ctx.close()
+            d.free()
+    return out
+
+
+def tune_mix(dtype="f32", nseg: int = 4096, reps: int = 8, rounds: int = 1, margin: float = 0.02,
+             path: str = DEFAULT_PATH, log=print) -> dict:
+    """Coordinate descent of the batch bands inside the mixed batch: for each
+    band of the workload, every candidate entry is tried with the rest of the
+    table fixed; the best is kept if the whole batch's replay is faster than
+    the current entry's by more than `margin`."""
+    pkg = _pkg()
+    dt = DTYPES[dtype]
+    apply(path)
+    spec = pkg.workloads.batch_spec(4096, nseg, 10, 22)
+    table = pkg.segment_table(spec["lengths"], spec["kinds"], spec["num_k"])
+    d = pkg.DeviceArray(int(spec["lengths"].sum()), dt)
+    for i, srow in enumerate(table):
+        pkg.generate(dt, "normal" if spec["normal"][i] else "uniform", int(spec["seeds"][i]),
+                     int(srow["length"]), out=d, offset=int(srow["offset"]))
+    bands = sorted({int(np.log2(float(n))) for n in spec["lengths"]})
+
+    def run():
+        t, res = time_batch(pkg, dt, d, table, spec["k"], reps)
+        return t, res
+
+    base_t, ref = run()
+    log(f"mix: start {base_t * 1e3:.3f} ms per replay", flush=True)
+    changed = {}
+    for _ in range(rounds):
+        for band in bands:
+            cur = pkg.tuning_get(dt, band)
+            best = (base_t, cur)
+            for bl, cl, direct in candidates(dt, band):
+                cand = (bl if not direct else 10, cl if not direct else 16, direct)
+                if cand == cur:
+                    continue
+                pkg.tuning_set(dt, band, *cand)
+                t, res = run()
+                if not np.array_equal(res.view(np.uint8), ref.view(np.uint8)):
+                    raise RuntimeError(f"mix band {band} {cand}: outputs differ")
+                if t < best[0]:
+                    best = (t, cand)
+            if best[1] != cur and best[0] < base_t * (1 - margin):
+                pkg.tuning_set(dt, band, *best[1])
+                base_t = best[0]
+                changed[str(band)] = list(best[1])
+                log(f"mix: band 2^{band} -> {best[1]}: {base_t * 1e3:.3f} ms", flush=True)
+            else:
+                pkg.tuning_set(dt, band, *cur)
+    table_now = {str(b): list(pkg.tuning_get(dt, b)) for b in bands}
+    d.free()
+    return {"ms_per_replay": base_t * 1e3, "changed": changed, "bands": table_now}
+
+
 def apply(path: str = DEFAULT_PATH) -> bool:
     """Load a tuned table into the library; False if there is none."""
     if not os.path.exists(path):
@@ -129,6 +256,9 @@ def apply(path: str = DEFAULT_PATH) -> bool:
     pkg = _pkg()
     with open(path) as f:
         doc = json.load(f)
+    for name, bands in doc.get("single", {}).items():
+        for band, (bl, ch) in bands.items():
+            pkg.tuning_single_set(DTYPES[name], int(band), int(bl), int(ch))
     for name, bands in doc.get("bands", {}).items():
         for band, (bl, cl, direct) in bands.items():
             pkg.tuning_set(DTYPES[name], int(band), int(bl), int(cl), bool(direct))
@@ -142,16 +272,35 @@ def apply(path: str = DEFAULT_PATH) -> bool:
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--dtypes", default="f32,f64")
-    ap.add_argument("--bands", default="10-26")
+    ap.add_argument("--mode", default="bands", choices=["bands", "single", "mix"])
+    ap.add_argument("--dtypes", default=None)
+    ap.add_argument("--bands", default=None)
     ap.add_argument("--elems", type=int, default=1 << 26)
     ap.add_argument("--out", default=DEFAULT_PATH)
     a = ap.parse_args()
-    b0, b1 = (int(v) for v in a.bands.split("-"))
     t0 = time.time()
-    res = tune(a.dtypes.split(","), range(b0, b1 + 1), a.elems)
-    doc = {"device": "gfx950", "method": "fastest bit-identical candidate per band, mean of "
-           "graph replays", "elems_per_band": a.elems, "bands": res}
+    doc = {}
+    if os.path.exists(a.out):
+        with open(a.out) as f:
+            doc = json.load(f)
+    doc["device"] = "gfx950"
+    if a.mode == "bands":
+        b0, b1 = (int(v) for v in (a.bands or "10-26").split("-"))
+        doc["bands"] = tune((a.dtypes or "f32,f64").split(","), range(b0, b1 + 1), a.elems)
+        doc["method"] = "fastest bit-identical candidate per band, mean of graph replays"
+        doc["elems_per_band"] = a.elems
+    elif a.mode == "single":
+        b0, b1 = (int(v) for v in (a.bands or "16-28").split("-"))
+        doc["single"] = tune_single((a.dtypes or "f32,f64,u32,i32").split(","), range(b0, b1 + 1))
+        doc["single_method"] = ("fastest bit-identical (buckets, chunks) per dtype and band, one "
+                                "resident vector of 2^band keys, 1024 uniform k, mean of "
+                                "back-to-back selects")
+    else:
+        r = tune_mix((a.dtypes or "f32"), path=a.out)
+        doc.setdefault("bands", {})[(a.dtypes or "f32")] = r["bands"]
+        doc["mix"] = {"ms_per_replay": round(r["ms_per_replay"], 4), "changed": r["changed"],
+                      "method": "coordinate descent of each band's entry inside the 4096-vector "
+                                "mixed batch, kept when the whole replay is > 2 % faster"}
     with open(a.out, "w") as f:
         json.dump(doc, f, indent=1)
     print(f"wrote {a.out} in {time.time() - t0:.0f}s", flush=True)

@@ -13,7 +13,7 @@ ROOT = os.path.dirname(HERE)
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = [os.path.join(HERE, "csrc", "ggks_select.hip")]
+SOURCES = [os.path.join(HERE, "csrc", "ggks_select.hip"), os.path.join(HERE, "csrc", "ggks_sort.hip")]
 HEADERS = [os.path.join(HERE, "csrc", h) for h in
            ("ggks_keys.hpp", "ggks_types.hpp", "ggks_kernels.hpp", "ggks_plan.hpp",
             "ggks_host.hpp", "ggks_batch.hpp", "ggks_serve.hpp", "ggks_gen.hpp")]

@@ -24,13 +24,32 @@ struct Tuning {
   uint32_t chunks;
 };
 
-inline Tuning default_tuning(uint64_t n, int eb) {
-  (void)eb;
+inline Tuning default_tuning(uint64_t n) {
   if (n >= (1ull << 26)) return {15, 256};
   if (n >= (1ull << 24)) return {13, 256};
   if (n >= (1ull << 22)) return {12, 128};
   if (n >= (1ull << 20)) return {11, 64};
   return {10, 32};
+}
+
+// Single-vector tuning per (dtype, floor(log2 n)): the defaults above until
+// the autotuner sets a band (ggks_tuning_single_set).
+constexpr int kSingleBands = 33;
+struct SingleTuningTable {
+  Tuning t[4][kSingleBands];
+  SingleTuningTable() {
+    for (int d = 0; d < 4; ++d)
+      for (int b = 0; b < kSingleBands; ++b) t[d][b] = default_tuning(1ull << b);
+  }
+};
+inline SingleTuningTable& single_tuning_table() {
+  static SingleTuningTable tab;
+  return tab;
+}
+inline Tuning single_tuning(int dtype, uint64_t n) {
+  int b = 0;
+  while (b < kSingleBands - 1 && (n >> (b + 1))) ++b;
+  return single_tuning_table().t[dtype][b];
 }
 
 // Bucket budget (log2) of a radix segment with digits of rb bits: room for
@@ -143,7 +162,7 @@ inline SingleShape single_shape(int kind, int dtype, uint64_t n, const ggks_opts
     sh.direct = true;
     return sh;
   }
-  Tuning t = default_tuning(n, eb);
+  Tuning t = single_tuning(dtype, n);
   uint32_t bl;
   sh.mode = MODE_SAMPLE;
   if (kind == GGKS_KIND_RADIX) {

@@ -478,6 +478,70 @@ static int select_one(ggks_ctx* c, int kind, int dtype, const void* d_vec, int64
 #include "ggks_batch.hpp"
 #include "ggks_serve.hpp"
 
+// ---------------------------------------------- sort-and-choose (baseline)
+// The reference's sort-and-choose (multiselectTimingFunctions.cu:41-95) on
+// the device: a full radix sort of a copy of the vector (ggks_sort.hip),
+// then out[i] = sorted[n - k_i].  The baseline the crossover driver races
+// bucket multiselect against (analyzeMultiselect.cu:209-274).
+size_t device_sort_temp_bytes(int dtype, uint64_t n);
+hipError_t device_sort(int dtype, const void* in, void* out, uint64_t n, void* temp,
+                       size_t temp_bytes, hipStream_t s);
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_pick_sorted(const T* __restrict__ sorted, uint64_t n,
+                                                     const uint32_t* __restrict__ k, uint32_t nk,
+                                                     T* __restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nk) out[i] = sorted[n - k[i]];
+}
+
+static int sort_choose(ggks_ctx* c, int dtype, const void* d_vec, int64_t n, const uint32_t* k,
+                       int64_t num_k, void* out) {
+  int rc = check_common(dtype, d_vec, n, num_k);
+  if (rc) return rc;
+  if (num_k > 0 && (!k || !out)) return fail(GGKS_EINVAL, "null k or output array");
+  for (int64_t i = 0; i < num_k; ++i)
+    if (k[i] < 1 || (int64_t)k[i] > n)
+      return fail(GGKS_EINVAL, "k[%lld]=%u outside [1, %lld]", (long long)i, k[i], (long long)n);
+  if (num_k == 0) return GGKS_OK;
+  if (!c && (rc = default_ctx(&c))) return rc;
+  std::lock_guard<std::mutex> lk(c->mu);
+  HIP_TRY(hipSetDevice(c->device));
+  const size_t eb = (size_t)dtype_bytes(dtype);
+  const size_t tb = device_sort_temp_bytes(dtype, (uint64_t)n);
+  if (!tb) return fail(GGKS_EDEVICE, "rocPRIM sort size query failed");
+  const size_t off_sorted = 0, off_temp = align_up((size_t)n * eb);
+  const size_t off_k = align_up(off_temp + tb), off_out = align_up(off_k + (size_t)num_k * 4);
+  if ((rc = ensure_arena(c, off_out + (size_t)num_k * eb))) return rc;
+  c->table_sig = 0;  // the plan tables' place in the arena is overwritten
+  const size_t pin_out = align_up((size_t)num_k * 4);
+  if ((rc = ensure_pinned(c, pin_out + (size_t)num_k * eb))) return rc;
+  if ((rc = pinned_claim(c))) return rc;
+  memcpy(c->pinned, k, (size_t)num_k * 4);
+  char* A = c->arena;
+  HIP_TRY(hipMemcpyAsync(A + off_k, c->pinned, (size_t)num_k * 4, hipMemcpyHostToDevice,
+                         c->stream));
+  c->pinned_busy = true;
+  HIP_TRY(device_sort(dtype, d_vec, A + off_sorted, (uint64_t)n, A + off_temp, tb, c->stream));
+  const uint32_t nk = (uint32_t)num_k, grid = (nk + 255) / 256;
+  const uint32_t* dk = reinterpret_cast<const uint32_t*>(A + off_k);
+  if (eb == 8)
+    hipLaunchKernelGGL(k_pick_sorted<uint64_t>, dim3(grid), dim3(256), 0, c->stream,
+                       reinterpret_cast<const uint64_t*>(A + off_sorted), (uint64_t)n, dk, nk,
+                       reinterpret_cast<uint64_t*>(A + off_out));
+  else
+    hipLaunchKernelGGL(k_pick_sorted<uint32_t>, dim3(grid), dim3(256), 0, c->stream,
+                       reinterpret_cast<const uint32_t*>(A + off_sorted), (uint64_t)n, dk, nk,
+                       reinterpret_cast<uint32_t*>(A + off_out));
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(c->pinned + pin_out, A + off_out, (size_t)num_k * eb,
+                         hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->pinned_busy = false;
+  memcpy(out, c->pinned + pin_out, (size_t)num_k * eb);
+  return GGKS_OK;
+}
+
 // ------------------------------------------------- no-allocation entry point
 // ggks_bucket_multiselect_noalloc runs the single-vector engine over a fixed
 // device array instead of a context arena: no hipMalloc, hipFree, pinned
@@ -742,6 +806,10 @@ int ggks_bucket_multiselect(ggks_ctx* c, int dtype, const void* d_vec, int64_t n
   return select_one(c, GGKS_KIND_BUCKET, dtype, d_vec, n, h_k, num_k, KSource::HOST, h_out, true,
                     opt);
 }
+int ggks_sort_choose(ggks_ctx* c, int dtype, const void* d_vec, int64_t n, const uint32_t* h_k,
+                     int64_t num_k, void* h_out) {
+  return sort_choose(c, dtype, d_vec, n, h_k, num_k, h_out);
+}
 int ggks_radix_multiselect(ggks_ctx* c, int dtype, const void* d_vec, int64_t n,
                            const uint32_t* h_k, int64_t num_k, void* h_out,
                            const ggks_opts* opt) {
@@ -938,6 +1006,26 @@ int ggks_tuning_get(int dtype, int log2_len, int* buckets_log2, int* chunk_log2,
 
 int ggks_tuning_reset(void) {
   tuning_table() = TuningTable();
+  single_tuning_table() = SingleTuningTable();
+  return GGKS_OK;
+}
+
+int ggks_tuning_single_set(int dtype, int log2_len, int buckets_log2, int chunks) {
+  if (dtype < 0 || dtype > 3 || log2_len < 0 || log2_len >= kSingleBands)
+    return fail(GGKS_EINVAL, "bad tuning band (dtype %d, log2 length %d)", dtype, log2_len);
+  if (buckets_log2 < 8 || buckets_log2 > 15 || chunks < 1 || chunks > (int)kMaxChunks)
+    return fail(GGKS_EINVAL, "bad single-vector tuning (buckets 2^%d, %d chunks)", buckets_log2,
+                chunks);
+  single_tuning_table().t[dtype][log2_len] = {(uint32_t)buckets_log2, (uint32_t)chunks};
+  return GGKS_OK;
+}
+
+int ggks_tuning_single_get(int dtype, int log2_len, int* buckets_log2, int* chunks) {
+  if (dtype < 0 || dtype > 3 || log2_len < 0 || log2_len >= kSingleBands)
+    return fail(GGKS_EINVAL, "bad tuning band (dtype %d, log2 length %d)", dtype, log2_len);
+  const Tuning t = single_tuning_table().t[dtype][log2_len];
+  if (buckets_log2) *buckets_log2 = (int)t.buckets_log2;
+  if (chunks) *chunks = (int)t.chunks;
   return GGKS_OK;
 }
 

@@ -137,6 +137,14 @@ int ggks_topk_u32(const uint32_t *d_vec, int64_t n, int32_t K, uint32_t *h_out,
 int ggks_topk_i32(const int32_t *d_vec, int64_t n, int32_t K, int32_t *h_out,
                   const ggks_opts *opt);
 
+/* The reference's sort-and-choose (multiselectTimingFunctions.cu:41-95,
+ * timeSortAndChooseMultiselect) on the device: a full radix sort of a copy of
+ * the vector (rocPRIM), then h_out[i] = sorted[n - h_k[i]].  The baseline
+ * bucket multiselect is raced against (analyzeMultiselect.cu:209-274); the
+ * input is left intact. */
+int ggks_sort_choose(ggks_ctx *ctx, int dtype, const void *d_vec, int64_t n,
+                     const uint32_t *h_k, int64_t num_k, void *h_out);
+
 /* ---- stream-ordered variants: device k list and device outputs ------
  * Nothing is synchronised and no host memory is touched, so these can be
  * captured in a hipGraph once the context's arena is large enough (call
@@ -283,6 +291,13 @@ int ggks_batch_destroy(ggks_batch *batch);
  * built-in defaults.  Not synchronised with concurrent batch creation. */
 int ggks_tuning_set(int dtype, int log2_len, int buckets_log2, int chunk_log2, int direct);
 int ggks_tuning_get(int dtype, int log2_len, int *buckets_log2, int *chunk_log2, int *direct);
+/* Single-vector table, per dtype and floor(log2 n): bucket budget 2^buckets_log2
+ * (8..15) and the number of chunks (work-groups of the streaming passes,
+ * 1..512) the single-vector entry points plan with when ggks_opts leaves
+ * them 0.  The autotuner measures and sets these.  ggks_tuning_reset
+ * restores the built-in defaults of both tables. */
+int ggks_tuning_single_set(int dtype, int log2_len, int buckets_log2, int chunks);
+int ggks_tuning_single_get(int dtype, int log2_len, int *buckets_log2, int *chunks);
 int ggks_tuning_reset(void);
 
 /* ---- serving loop ----------------------------------------------------
