@@ -29,7 +29,7 @@ EXPORTS = [
     "ggks_topk_i32",
     "ggks_bucket_multiselect_async", "ggks_radix_multiselect_async", "ggks_topk_async",
     "ggks_ctx_reserve", "ggks_ctx_check", "ggks_ctx_stream", "ggks_dev_alloc",
-    "ggks_bucket_multiselect_noalloc", "ggks_sort_choose",
+    "ggks_bucket_multiselect_noalloc", "ggks_sort_choose", "ggks_ctx_debug_regions",
     "ggks_dev_free", "ggks_h2d", "ggks_d2h", "ggks_d2d", "ggks_sync",
     "ggks_device_count", "ggks_set_device", "ggks_dev_fill", "ggks_ctx_profile_arm",
     "ggks_ctx_profile_read", "ggks_ctx_profile_marks", "ggks_batch_create", "ggks_batch_scratch_bytes", "ggks_batch_run",
@@ -101,6 +101,15 @@ class Request(ctypes.Structure):
 assert REQUEST_DTYPE.itemsize == ctypes.sizeof(Request)
 
 
+class DebugRegions(ctypes.Structure):
+    """ggks_debug_regions (include/ggks.h)."""
+    _fields_ = [(f, ctypes.c_void_p) for f in ("segs", "states", "lut", "chunks", "rows", "totals",
+                                              "gsums", "bits", "children", "cq", "red")] + [
+        ("n", ctypes.c_uint64), ("nchunks", ctypes.c_uint32), ("bmax", ctypes.c_uint32),
+        ("nq", ctypes.c_uint32), ("pbits", ctypes.c_uint32), ("dtype", ctypes.c_int32),
+        ("direct", ctypes.c_int32)]
+
+
 class ServerStats(ctypes.Structure):
     """ggks_server_stats (include/ggks.h)."""
     _fields_ = [(f, ctypes.c_uint64) for f in (
@@ -137,6 +146,7 @@ def lib() -> ctypes.CDLL:
         "ggks_ctx_reserve": ([vp, i32, i32, i64, i64], i32),
         "ggks_bucket_multiselect_noalloc": ([i32, vp, i64, vp, i64, vp, optp, vp], i32),
         "ggks_sort_choose": ([vp, i32, vp, i64, vp, i64, vp], i32),
+        "ggks_ctx_debug_regions": ([vp, ctypes.POINTER(DebugRegions)], i32),
         "ggks_ctx_check": ([vp], i32),
         "ggks_ctx_stream": ([vp], vp),
         "ggks_dev_alloc": ([ctypes.POINTER(vp), sz], i32),

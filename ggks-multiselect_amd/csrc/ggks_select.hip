@@ -67,6 +67,7 @@ struct ggks_ctx {
   uint32_t prof_armed = ~0u;  // the mask latched by the last arm: recorded and read
   // plan-table cache: the tables in the arena match this signature
   uint64_t table_sig = 0;
+  ggks_debug_regions dbg = {};  // the last select's tables (ggks_ctx_debug_regions)
   bool pinned_busy = false;  // an async copy from `pinned` may be in flight
 };
 
@@ -461,6 +462,31 @@ static int select_one(ggks_ctx* c, int kind, int dtype, const void* d_vec, int64
   void* d_out = host_out ? (void*)(c->arena + off_dout) : out;
   const Exec x = {c->arena, c->stream, c->status, c->prof_cur >= 0 ? c : nullptr};
   rc = launch(x, dtype, p, d_vec, d_out, d_k);
+  {
+    ggks_debug_regions& g = c->dbg;
+    g = ggks_debug_regions{};
+    g.n = (uint64_t)n;
+    g.dtype = dtype;
+    g.direct = p.direct ? 1 : 0;
+    g.nq = p.nq;
+    if (!p.direct) {
+      char* A = c->arena;
+      g.segs = A + p.off_segs;
+      g.states = A + p.off_states;
+      g.lut = A + p.off_lut;
+      g.chunks = A + p.off_chunks;
+      g.rows = A + p.off_rows;
+      g.totals = A + p.off_tot;
+      g.gsums = A + p.off_gsum;
+      g.bits = A + p.off_bits;
+      g.children = A + p.off_children;
+      g.cq = A + p.off_cq;
+      g.red = A + p.off_red;
+      g.nchunks = p.nchunks;
+      g.bmax = p.bmax_max;
+      g.pbits = p.pbits_max;
+    }
+  }
   mark(c, GGKS_NSTAGES);
   c->prof_cur = -1;
   if (rc) return rc;
@@ -702,6 +728,13 @@ int ggks_ctx_destroy(ggks_ctx* c) {
 }
 
 void* ggks_ctx_stream(ggks_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int ggks_ctx_debug_regions(ggks_ctx* c, ggks_debug_regions* out) {
+  if (!c || !out) return fail(GGKS_EINVAL, "null argument");
+  std::lock_guard<std::mutex> lk(c->mu);
+  *out = c->dbg;
+  return GGKS_OK;
+}
 
 int ggks_ctx_profile_arm(ggks_ctx* c, int nslots) {
   if (!c || nslots < 0) return fail(GGKS_EINVAL, "bad profile arguments");

@@ -187,6 +187,24 @@ int ggks_ctx_reserve(ggks_ctx *ctx, int kind, int dtype, int64_t n,
 int ggks_ctx_check(ggks_ctx *ctx);
 void *ggks_ctx_stream(ggks_ctx *ctx);
 
+/* ---- inspection (tests and diagnostics) -----------------------------
+ * Device addresses of the intermediate tables of the last engine select on
+ * a context (valid after its stream drained, until the next call on the
+ * context): the segment descriptor and state (SegState of the bucket
+ * function), its LUT, the chunk table, the per-chunk rows (after the reduce: each chunk's
+ * exclusive offset inside every bucket), the bucket totals (prefix inside
+ * each group of 64 buckets) and group sums, the active-bucket bitmap, the
+ * child records and their queries, and the reduced buffer.  Layouts are
+ * csrc/ggks_types.hpp's; direct = 1 means the last select had no engine
+ * segment (no tables). */
+typedef struct ggks_debug_regions {
+  const void *segs, *states, *lut, *chunks, *rows, *totals, *gsums, *bits, *children, *cq, *red;
+  uint64_t n;
+  uint32_t nchunks, bmax, nq, pbits;
+  int32_t dtype, direct;
+} ggks_debug_regions;
+int ggks_ctx_debug_regions(ggks_ctx *ctx, ggks_debug_regions *out);
+
 /* ---- stage timing ---------------------------------------------------
  * When armed with nslots > 0, every select on the context records HIP events
  * on its stream around each kernel stage into slot (call index % nslots).

@@ -61,3 +61,61 @@ def test_adversarial_2p27_cluster_k(pkg, O, case):
     k = pkg.workloads.k_cluster(103, n)
     for kind in ("bucket", "radix"):
         check(pkg, O, d, x, k, kind=kind, label=case)
+
+
+def test_c4_batch_full_shape_every_segment(pkg, O):
+    """The 4096-vector uneven batch at its full shape (lengths floor(2^U),
+    U ~ U[10, 22], ~2.2e9 float32 keys, every band up to 2^21 present),
+    graph replay, every segment checked against the oracle (its radix
+    sort-and-choose or top-K, one C call per segment in a thread pool)."""
+    from concurrent.futures import ThreadPoolExecutor
+    spec = pkg.workloads.batch_spec(4096, 4096, 10, 22)
+    table = pkg.segment_table(spec["lengths"], spec["kinds"], spec["num_k"])
+    N = int(spec["lengths"].sum())
+    bands = {int(np.log2(float(n))) for n in spec["lengths"]}
+    assert 21 in bands and 10 in bands
+    print(f"case c4 full shape: {len(table)} segments, {N} keys", flush=True)
+    pkg.autotune.apply()
+    d = pkg.DeviceArray(N, np.float32)
+    for i, srow in enumerate(table):
+        pkg.generate(np.float32, "normal" if spec["normal"][i] else "uniform", int(spec["seeds"][i]),
+                     int(srow["length"]), out=d, offset=int(srow["offset"]))
+    b = pkg.Batch(np.float32, table, spec["k"])
+    got = b(d, graph=True)
+    got2 = b(d, graph=True)
+    assert np.array_equal(bits(got), bits(got2))
+    x = d.to_host()
+    k = spec["k"]
+
+    def seg_ok(i):
+        s = table[i]
+        seg = x[int(s["offset"]):int(s["offset"] + s["length"])]
+        if s["kind"] == 2:
+            want = O.topk(seg, int(s["num_k"]))
+        else:
+            want = O.sort_choose(seg, k[int(s["k_offset"]):int(s["k_offset"] + s["num_k"])])
+        g = got[int(s["out_offset"]):int(s["out_offset"] + s["num_k"])]
+        return bool(np.array_equal(bits(g), bits(want)))
+
+    with ThreadPoolExecutor(16) as ex:
+        ok = list(ex.map(seg_ok, range(len(table))))
+    bad = [i for i, v in enumerate(ok) if not v]
+    assert not bad, f"{len(bad)} segments wrong, first {bad[:5]}"
+    b.close()
+    d.free()
+
+
+def test_c5_pool_full_shape_trace(pkg):
+    """A served trace over the serving workload's full pool (64 resident
+    vectors, lengths log-uniform up to 2^26, f32 / u32 / f64), every answer
+    checked against sorted host copies."""
+    S = pkg.serving
+    pool = S.make_pool(5000, 64)
+    assert pool["lengths"].max() > 1 << 25
+    bufs = S.upload_pool(pool)
+    host = S.sorted_copies(pool, bufs)
+    srv = pkg.Server(bufs, scratch_budget=32 << 30, max_inflight=4, max_batch=256)
+    print("case c5 full pool, 8k req/s for 1 s", flush=True)
+    r = S.run(srv, pool, host, 8000.0, 1.0, seed=5001)
+    assert r["completed"] == r["requests"] > 5000 and r["bit_exact"], r
+    srv.close()
