@@ -297,7 +297,8 @@ def main():
                                 "back-to-back selects")
     else:
         r = tune_mix((a.dtypes or "f32"), path=a.out)
-        doc.setdefault("bands", {})[(a.dtypes or "f32")] = r["bands"]
+        # the mixed batch spans bands 10..21: the other bands keep their entries
+        doc.setdefault("bands", {}).setdefault(a.dtypes or "f32", {}).update(r["bands"])
         doc["mix"] = {"ms_per_replay": round(r["ms_per_replay"], 4), "changed": r["changed"],
                       "method": "coordinate descent of each band's entry inside the 4096-vector "
                                 "mixed batch, kept when the whole replay is > 2 % faster"}

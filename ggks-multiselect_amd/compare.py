@@ -98,6 +98,9 @@ def compare(pkg, n: int, num_k: int, dtype=np.float32, family: str = "uniform",
         d = pkg.generate(dtype, family, seed + t, n, hi=n if family == "sorted" else 0)
         k = k_list(W, kfamily, seed + 1000 + t, num_k, n)
         res, ms = {}, {}
+        if t == 0:  # untimed: first-call costs (scratch growth, library set-up)
+            for a in algorithms:
+                _run(pkg, ALGORITHMS[a], d, k)
         for j in rng.permutation(len(algorithms)):
             a = algorithms[j]
             res[a], ms[a] = _run(pkg, ALGORITHMS[a], d, k)
