@@ -65,10 +65,13 @@ def parse():
                     help="length of each served trace (the workload definition uses 60)")
     ap.add_argument("--serve-rates", default="2000,8000,32000")
     ap.add_argument("--batch-segments", type=int, default=4096)
+    ap.add_argument("--batch-no-loop", action="store_true",
+                    help="batched leg: graph replays only, no looped single-vector calls "
+                         "(for counter runs, where every dispatch is serialised)")
     ap.add_argument("--no-adversarial", action="store_true", help="skip the adversarial leg")
     ap.add_argument("--no-topk", action="store_true", help="skip the top-K and radix legs")
     ap.add_argument("--no-saturation", action="store_true", help="skip the saturation sweep")
-    ap.add_argument("--saturation-rates", default="64000,128000,256000,512000")
+    ap.add_argument("--saturation-rates", default="32000,40000,48000,56000,64000,96000,128000")
     ap.add_argument("--saturation-seconds", type=float, default=2.0)
     ap.add_argument("--p99-bound-ms", type=float, default=5.0,
                     help="latency bound of the saturation sweep")
@@ -268,7 +271,7 @@ def adversarial_leg(pkg, L, steps):
             "inputs": out}
 
 
-def batched_leg(pkg, L, nseg: int):
+def batched_leg(pkg, L, nseg: int, loop_too: bool = True):
     """4096 uneven f32 vectors (lengths floor(2^U), U ~ U[10, 22]; uniform or
     normal values; 1..64 k or a top-K each) selected by one batch (graph
     replay) against looping the single-vector entry point over the same
@@ -299,6 +302,9 @@ def batched_leg(pkg, L, nseg: int):
     b.sync()
     t_batch = timed(lambda: b.run(d, out, graph=True), 10)
     got = out.to_host()
+    if not loop_too:
+        return {"batched_ms": round(t_batch * 1e3, 3), "elements": N,
+                "batched_gelem_per_s": round(N / t_batch / 1e9, 2)}
     ctx = pkg.Context(0)
     dk = pkg.DeviceArray.from_host(spec["k"] if spec["k"].size else np.ones(1, np.uint32))
     lout = pkg.DeviceArray(b.num_outputs, np.float32)
@@ -582,7 +588,7 @@ def main():
         out["f64"] = f64_leg(pkg, L, a.steps)
     if not a.no_batch:
         print("# batched leg", file=sys.stderr, flush=True)
-        out["batched"] = batched_leg(pkg, L, a.batch_segments)
+        out["batched"] = batched_leg(pkg, L, a.batch_segments, not a.batch_no_loop)
     if not a.no_serve:
         print("# serving leg", file=sys.stderr, flush=True)
         out["serving"] = serving_leg(pkg, [float(r) for r in a.serve_rates.split(",")],

@@ -177,6 +177,69 @@ __device__ __forceinline__ void lds_bitonic_sort(KT* keys, uint32_t np) {
   }
 }
 
+// Ascending LSD radix sort of keys[0, np) in LDS by a work-group of
+// kThreads (np a multiple of kThreads, at most S * kThreads), on the low
+// nbits bits (every key below 2^nbits, or padding equal to the all-ones key,
+// which sorts last), 8-bit digits.  The keys live in registers between
+// passes: wavefront w holds the tile [w * np / 16, (w + 1) * np / 16), lane
+// l its keys w * np / 16 + 64 s + l; each pass counts digits per
+// (digit, wavefront) in whist (256 * 16 words), scans the counts in that
+// order (so equal digits keep tile order: the sort is stable) and scatters
+// every key to its place in LDS, which the next pass reads back.  A key's
+// place among its wavefront's equal digits in one step is the number of
+// lower lanes with the same digit, from eight ballots over the digit's bits.
+// Far fewer barriers than a bitonic sort (3 per pass against 91 stages for
+// 8 Ki keys).
+template <typename KT, uint32_t S>
+__device__ void lds_radix_sort(KT* keys, uint32_t np, int nbits, uint32_t* whist,
+                               uint32_t* wsum) {
+  constexpr uint32_t NW = kThreads / 64;
+  const uint32_t tid = threadIdx.x, lane = lane_id(), w = tid >> 6;
+  const uint32_t steps = np / kThreads;  // keys per lane
+  const uint32_t tile = w * (np / NW);
+  const uint64_t below = (1ull << lane) - 1ull;
+  KT k[S];
+#pragma unroll
+  for (uint32_t t = 0; t < S; ++t)
+    if (t < steps) k[t] = keys[tile + t * 64 + lane];
+  for (int sh = 0; sh < nbits; sh += 8) {
+    for (uint32_t i = tid; i < 256 * NW; i += kThreads) whist[i] = 0;
+    __syncthreads();
+#pragma unroll
+    for (uint32_t t = 0; t < S; ++t)
+      if (t < steps) atomicAdd(&whist[(uint32_t)((k[t] >> sh) & 0xFF) * NW + w], 1u);
+    __syncthreads();
+    block_scan_array(whist, 256 * NW, wsum);  // whist[d * NW + w] = first place (synchronises)
+#pragma unroll
+    for (uint32_t t = 0; t < S; ++t) {
+      if (t < steps) {  // uniform
+        const uint32_t d = (uint32_t)((k[t] >> sh) & 0xFF);
+        uint64_t peers = ~0ull;
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+          const uint64_t m = __ballot((d >> b) & 1u);
+          peers &= ((d >> b) & 1u) ? m : ~m;
+        }
+        const uint32_t rank = (uint32_t)__popcll(peers & below);
+        uint32_t off = 0;
+        if (rank == 0) {  // the lowest lane of the digit's peers claims their places
+          off = whist[d * NW + w];
+          whist[d * NW + w] = off + (uint32_t)__popcll(peers);
+        }
+        off = (uint32_t)__shfl((int)off, __ffsll((unsigned long long)peers) - 1, 64);
+        keys[off + rank] = k[t];
+      }
+    }
+    __syncthreads();
+    if (sh + 8 < nbits) {
+#pragma unroll
+      for (uint32_t t = 0; t < S; ++t)
+        if (t < steps) k[t] = keys[tile + t * 64 + lane];
+    }
+  }
+  __syncthreads();
+}
+
 template <typename KT>
 __device__ __forceinline__ uint32_t mulhi_nb(KT x, uint32_t nb);
 template <>
@@ -473,12 +536,20 @@ __global__ __launch_bounds__(kSampleThreads) void k_sample(const KT* __restrict_
                                                            const SegDesc* __restrict__ segs,
                                                            const SampleBlock* __restrict__ sblocks,
                                                            KT* __restrict__ samples,
-                                                           KT* __restrict__ minmax, uint64_t seed) {
+                                                           KT* __restrict__ minmax, uint64_t seed,
+                                                           SegState* __restrict__ states,
+                                                           uint32_t* __restrict__ lut_pool) {
   __shared__ KT red_lo[kSampleWaves], red_hi[kSampleWaves], cand[kSampleWaves];
   __shared__ uint32_t ccnt[kSampleWaves][kSampleWaves];  // [wave][candidate]
   const SampleBlock sb = sblocks[blockIdx.x];
   const SegDesc& sd = segs[sb.seg];
   if (sd.mode == MODE_RANGE || sb.blk * kThreads >= sd.nsample) return;  // uniform
+  if (sb.blk == 0 && lut_parts(sd.mode, sd.nsample, sizeof(KT)) > 1) {
+    // the split LUT build adds its partial sample histograms into the LUT
+    // region and counts its parts in the state: both start at zero
+    for (uint32_t i = threadIdx.x; i < (1u << sd.pbits); i += kSampleThreads) lut_pool[sd.lut_off + i] = 0;
+    if (threadIdx.x == 0) states[sb.seg].arrive = 0;
+  }
   const uint32_t w = threadIdx.x >> 6, lane = lane_id();
   KT k[kSamplePer];
 #pragma unroll
@@ -543,6 +614,32 @@ __global__ __launch_bounds__(kSampleThreads) void k_sample(const KT* __restrict_
   }
 }
 
+// Finisher shapes that k_findk plans for.  k_finish_small takes children of
+// at most kWaveRanks ranks with a work-group of kFThreads.
+constexpr uint32_t kWaveRanks = 4;
+constexpr uint32_t kFThreads = 256;
+// 16-byte words per thread held by k_finish_small: 36 KB of 4-byte keys,
+// 32 KB of 8-byte keys in registers (the most that fit in 128 VGPRs without
+// spilling for every key type), and kFLdsWords more per thread in LDS (28 KB
+// per work-group: with the rest of its LDS, four work-groups still share a
+// CU), so children up to 64 KB of 4-byte keys and 60 KB of 8-byte keys are
+// read from memory once.  At 2^28 float32 keys with 2^15 buckets about one
+// child in eight is longer than the registers alone hold (mean 8.4 K keys,
+// largest 13 K); at 2^27 doubles about half (mean 4.5 K).
+template <typename KT>
+constexpr int kFRegWords = sizeof(KT) == 4 ? 9 : 8;
+constexpr int kFLdsWords = 7;
+// ... and by k_finish_small_lite (4-byte keys): 4 words (16 KB of keys) in at
+// most 80 VGPRs and none in LDS, so that six work-groups share a CU instead
+// of four (plans whose children are small: batches of many segments,
+// mostly).  The 8-byte instance would spill at that size and is not built.
+constexpr int kFLiteWords = 4;
+template <typename KT>
+constexpr uint64_t finish_small_reg_keys(bool lite) {
+  return (uint64_t)(lite ? kFLiteWords : kFRegWords<KT> + kFLdsWords) * kFThreads *
+         (16 / sizeof(KT));
+}
+
 // A sampled key is a heavy value when it is at least this many of one
 // block's kThreads samples (3.1 %): it then gets a bucket of its own.
 constexpr uint32_t kHeavyMinCount = 32;
@@ -568,15 +665,19 @@ __global__ __launch_bounds__(kThreads) void k_lut(const SegDesc* __restrict__ se
                                                   uint32_t* __restrict__ l2count) {
   // the second level's counters (heavy children, chunks) start at zero in
   // every run; k_findk, which counts them, runs after this kernel
-  if (l2count && blockIdx.x == 0 && threadIdx.x < 2) l2count[threadIdx.x] = 0;
+  if (l2count && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < 2) l2count[threadIdx.x] = 0;
   __shared__ uint32_t cnt[1u << kMaxPrefixBits];
   __shared__ uint32_t wsum[17];
   __shared__ KT red_lo[16], red_hi[16];
   __shared__ KT sh_hv;
-  __shared__ uint32_t sh_hvon;
+  __shared__ uint32_t sh_hvon, sh_last;
   constexpr uint32_t W = KeyBits<KT>::W;
   const SegDesc sd = segs[blockIdx.x];
   const uint32_t tid = threadIdx.x;
+  // part blockIdx.y of the segment's LUT build: its parts, at most the
+  // grid's y extent (1 keeps every segment's build in one work-group)
+  const uint32_t nparts = min(lut_parts(sd.mode, sd.nsample, sizeof(KT)), gridDim.y), part = blockIdx.y;
+  if (part >= nparts) return;  // uniform
   uint32_t* lut = lut_pool + sd.lut_off;
   KT kmin, kmax;
   uint32_t pb = sd.pbits;
@@ -679,11 +780,34 @@ __global__ __launch_bounds__(kThreads) void k_lut(const SegDesc* __restrict__ se
   } else {
     for (uint32_t p = tid; p < P; p += kThreads) cnt[p] = 0;
     __syncthreads();
-    // the whole sample in one round of loads (64 per thread for 4-byte keys)
-    strided_for<64 * 4 / sizeof(KT)>(samples + sd.sample_off, sd.nsample, [&](uint32_t, KT k) {
-      atomicAdd(&cnt[(uint32_t)((KT)(k - kb) >> s)], 1u);
-    });
-    __syncthreads();
+    if (nparts == 1) {
+      // the whole sample in one round of loads (64 per thread for 4-byte keys)
+      strided_for<64 * 4 / sizeof(KT)>(samples + sd.sample_off, sd.nsample, [&](uint32_t, KT k) {
+        atomicAdd(&cnt[(uint32_t)((KT)(k - kb) >> s)], 1u);
+      });
+      __syncthreads();
+    } else {
+      // this part's share of the sample; the partial counts are added into
+      // the segment's LUT region (zeroed by k_sample), and the last part to
+      // arrive goes on with their sums
+      const uint32_t per = sd.nsample / nparts;  // the last part takes the remainder
+      const uint32_t mine = part + 1 < nparts ? per : sd.nsample - part * per;
+      strided_for<kLutPartBytes / sizeof(KT) / kThreads>(samples + sd.sample_off + part * per, mine,
+                                              [&](uint32_t, KT k) {
+        atomicAdd(&cnt[(uint32_t)((KT)(k - kb) >> s)], 1u);
+      });
+      __syncthreads();
+      for (uint32_t p = tid; p < P; p += kThreads)
+        if (cnt[p]) atomicAdd(&lut[p], cnt[p]);
+      __threadfence();  // release: the sums before the arrival count
+      __syncthreads();
+      if (tid == 0) sh_last = atomicAdd(&states[blockIdx.x].arrive, 1u) == nparts - 1 ? 1u : 0u;
+      __syncthreads();
+      if (!sh_last) return;  // uniform
+      __threadfence();  // acquire: every part's sums
+      for (uint32_t p = tid; p < P; p += kThreads) cnt[p] = __hip_atomic_load(&lut[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __syncthreads();
+    }
     // prefixes holding no sample: a run of them shares one bucket
     uint32_t nonempty = 0, runs = 0;
     for (uint32_t p = tid; p < P; p += kThreads) {
@@ -1065,7 +1189,9 @@ __device__ __forceinline__ void findk_segment(const SegDesc* __restrict__ segs,
     wbase_pool[sd.bits_off + tid] = ubase;
   }
   __syncthreads();
-  // one thread per child: its bucket, reduced slice and key range
+  const bool l2 = (sd.flags & SEG_LEVEL2) && l2count;
+  // one thread per child: its bucket and reduced slice (its key range once
+  // its ranks are counted, below)
   for (uint32_t u = tid; u < U; u += kThreads) {
     uint32_t a = 0, z = nwords - 1;  // largest word with wb <= u
     while (a < z) {
@@ -1081,14 +1207,12 @@ __device__ __forceinline__ void findk_segment(const SegDesc* __restrict__ segs,
     Child c;
     c.start = sd.red_base + start;
     c.len = count_of(b);
-    KT klo, khi;
-    range(b, klo, khi);
-    c.lo = klo;
-    c.hi = khi;
+    c.lo = 0;
+    c.hi = 0;
+    c.need_range = 1;
     c.q_off = 0;
     c.q_cnt = 0;
     c.src = 1;
-    c.need_range = 0;
     c.bucket = b;
     c.seg = blockIdx.x;
     children[sd.q_off + u] = c;
@@ -1108,7 +1232,6 @@ __device__ __forceinline__ void findk_segment(const SegDesc* __restrict__ segs,
   }
   __syncthreads();
   const uint32_t nq = block_scan_array(hist, U, wsum);  // hist[u] = first slot
-  const bool l2 = (sd.flags & SEG_LEVEL2) && l2count;
   for (uint32_t u = tid; u < U; u += kThreads) {
     uint32_t first = hist[u];
     uint32_t next = (u + 1 < U) ? hist[u + 1] : nq;
@@ -1116,6 +1239,19 @@ __device__ __forceinline__ void findk_segment(const SegDesc* __restrict__ segs,
     cp->q_off = sd.q_off + first;
     cp->q_cnt = next - first;
     const uint64_t clen = cp->len;
+    // the key range where a later step needs it before reading the keys (a
+    // child the second level may take, or one k_finish walks from memory);
+    // a child of few ranks held in the 256-thread finisher's registers gets
+    // the exact min and max of its keys there, one pass over registers,
+    // which is cheaper than inverting the bucket function here
+    if ((l2 && clen > kHeavyKeys) || clen > finish_small_reg_keys<KT>(false) ||
+        next - first > kWaveRanks) {
+      KT klo, khi;
+      range(cp->bucket, klo, khi);
+      cp->lo = klo;
+      cp->hi = khi;
+      cp->need_range = 0;
+    }
     if (l2 && clen > kHeavyKeys) {
       // a heavy child goes to the second level: a heavy-child record and its
       // chunks; its slot finishes nothing until k_count2 writes a
@@ -1684,7 +1820,6 @@ __global__ __launch_bounds__(kThreads) void k_copy16(const uint4* __restrict__ s
 // Children with few ranks are finished by k_finish_small (one 256-thread
 // work-group each); the others by k_finish (a 1024-thread work-group with
 // the child in LDS).
-constexpr uint32_t kWaveRanks = 4;
 constexpr uint64_t kWaveMaxLen = 1ull << 15;
 __device__ __forceinline__ bool wave_child(const Child& c) {
   return c.q_cnt <= kWaveRanks && c.len <= kWaveMaxLen;
@@ -1699,7 +1834,6 @@ __device__ __forceinline__ bool sort_child(const Child& c) {
 // k_finish sorts a child held in LDS when it has many ranks or is short
 // enough that a sort beats the digit passes.
 constexpr uint32_t kSortLen = 4096;
-constexpr uint32_t kFThreads = 256;
 constexpr uint32_t kFDigit = 11;                      // first digit: 2048 bins
 constexpr uint32_t kFWords = (1u << kFDigit) + (1u << kFDigit) / 32;  // padded: bin b at b + b/32
 constexpr uint32_t kFCollectWords = 1024;             // keys gathered after the first digit
@@ -1738,32 +1872,46 @@ __device__ __forceinline__ void group_for_each(const KT* __restrict__ p, uint64_
   if (done + tid < len) f(p[done + tid]);
 }
 
-// A child's keys held in registers by a work-group of kFThreads: R 16-byte
-// words per thread plus at most one unaligned head key and one tail key, so
-// a child of up to R * kFThreads * 16 bytes is read from memory once for all
-// passes.  Visiting order differs from group_for_each; no pass depends on it.
-template <typename KT, int R>
+// A child's keys held by a work-group of kFThreads: R 16-byte words per
+// thread in registers and X more in LDS (each thread's own slots of `ext`,
+// X * kFThreads words: no other thread reads them), plus at most one
+// unaligned head key and one tail key, so a child of up to (R + X) *
+// kFThreads * 16 bytes is read from memory once for all passes.  Visiting
+// order differs from group_for_each; no pass depends on it.
+template <typename KT, int R, int X = 0>
 struct GroupKeys {
   static constexpr uint32_t VEC = 16 / sizeof(KT);
   uint4 v[R];
+  uint4* ext;
   KT hk, tk;
-  uint32_t nv;  // valid words of this thread: v[0, nv)
+  uint32_t nv;  // valid words of this thread: v[0, min(nv, R)), then ext
   bool hh, ht;
   // whether a child at p of len keys fits (uniform)
   __device__ static bool fits(const KT* p, uint64_t len) {
     const uint64_t mis = ((uintptr_t)p & 15u) / sizeof(KT);
     uint64_t head = mis ? (VEC - mis) : 0;
     if (head > len) head = len;
-    return (len - head) / VEC <= (uint64_t)R * kFThreads;
+    return (len - head) / VEC <= (uint64_t)(R + X) * kFThreads;
   }
-  __device__ void load(const KT* __restrict__ p, uint64_t len) {
+  __device__ void load(const KT* __restrict__ p, uint64_t len, uint4* lds_ext = nullptr) {
     const uint32_t tid = threadIdx.x;
     const uint64_t mis = ((uintptr_t)p & 15u) / sizeof(KT);
     uint64_t head = mis ? (VEC - mis) : 0;
     if (head > len) head = len;
-    const uint32_t nvec = (uint32_t)((len - head) / VEC);  // <= R * kFThreads
+    const uint32_t nvec = (uint32_t)((len - head) / VEC);  // <= (R + X) * kFThreads
     const uint4* __restrict__ pv = reinterpret_cast<const uint4*>(p + head);
     nv = nvec > tid ? (nvec - tid + kFThreads - 1) / kFThreads : 0u;
+    ext = lds_ext;
+    if (X > 0 && nv > (uint32_t)R) {
+      // the LDS words first, stored before the register words are loaded, so
+      // the two sets are never live in registers together (a thread with no
+      // LDS words skips this; only children longer than the registers hold
+      // pay the second round trip)
+#pragma unroll
+      for (int x = 0; x < X; ++x)
+        if ((uint32_t)(R + x) < nv) ext[x * kFThreads + tid] = pv[tid + (R + x) * kFThreads];
+      __asm__ __volatile__("" ::: "memory");
+    }
 #pragma unroll
     for (int u = 0; u < R; ++u) v[u] = (uint32_t)u < nv ? pv[tid + u * kFThreads] : make_uint4(0, 0, 0, 0);
     hh = tid < head;
@@ -1775,34 +1923,28 @@ struct GroupKeys {
   template <typename F>
   __device__ __forceinline__ void for_each(F&& f) const {
     if (hh) f(hk);
-#pragma unroll
-    for (int u = 0; u < R; ++u) {
-      if ((uint32_t)u < nv) {
-        if (sizeof(KT) == 4) {
-          f((KT)v[u].x); f((KT)v[u].y); f((KT)v[u].z); f((KT)v[u].w);
-        } else {
-          f((KT)(((uint64_t)v[u].y << 32) | v[u].x));
-          f((KT)(((uint64_t)v[u].w << 32) | v[u].z));
-        }
+    auto apply = [&](const uint4& w) {
+      if (sizeof(KT) == 4) {
+        f((KT)w.x); f((KT)w.y); f((KT)w.z); f((KT)w.w);
+      } else {
+        f((KT)(((uint64_t)w.y << 32) | w.x));
+        f((KT)(((uint64_t)w.w << 32) | w.z));
       }
+    };
+#pragma unroll
+    for (int u = 0; u < R; ++u)
+      if ((uint32_t)u < nv) apply(v[u]);
+    if (X > 0 && nv > (uint32_t)R) {
+      // read the LDS words afresh in every pass: kept live across passes
+      // (the compiler may merge the identical reads) they would spill
+      __asm__ __volatile__("" ::: "memory");
+#pragma unroll
+      for (int x = 0; x < X; ++x)
+        if ((uint32_t)(R + x) < nv) apply(ext[x * kFThreads + threadIdx.x]);
     }
     if (ht) f(tk);
   }
 };
-// 16-byte words per thread held by k_finish_small: children up to 36 KB of
-// 4-byte keys, 32 KB of 8-byte keys (the most that fit in 128 VGPRs without
-// spilling for every key type)
-template <typename KT>
-constexpr int kFRegWords = sizeof(KT) == 4 ? 9 : 8;
-// ... and by k_finish_small_lite (4-byte keys): 4 words (16 KB of keys) in at
-// most 80 VGPRs, so that six work-groups share a CU instead of four (plans
-// whose children are small: batches of many segments, mostly).  The 8-byte
-// instance would spill at that size and is not built.
-constexpr int kFLiteWords = 4;
-template <typename KT>
-constexpr uint64_t finish_small_reg_keys(bool lite) {
-  return (uint64_t)(lite ? kFLiteWords : kFRegWords<KT>) * kFThreads * (16 / sizeof(KT));
-}
 
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1841,7 +1983,7 @@ __device__ __forceinline__ void wave_pick256(const uint32_t* h, uint32_t lane, u
 // beyond it (heavy duplicates of one digit) by 8-bit radix passes over the
 // child in memory.  The second kind is sorted in LDS and every rank read off.
 // R is the number of 16-byte words of a child a thread holds in registers.
-template <typename KT, int C, int R>
+template <typename KT, int C, int R, int X>
 __device__ __forceinline__ void finish_small_body(const KT* __restrict__ input,
                                                   const KT* __restrict__ red,
                                                   const KT* __restrict__ red2,
@@ -1861,6 +2003,7 @@ __device__ __forceinline__ void finish_small_body(const KT* __restrict__ input,
   KT* buf = reinterpret_cast<KT*>(h);  // gathered keys (after the histogram is read)
   uint32_t* bins = h + kFCollectWords;  // 256 bins for the LDS radix passes
   __shared__ uint32_t list[kFThreads + 1];  // this window's children for this finisher; count
+  __shared__ uint4 gk_ext[X > 0 ? X * kFThreads : 1];  // the child's keys past the registers
   // this work-group's children are blockIdx.x, + gridDim.x, ...; they are
   // screened kFThreads at a time, one per thread, and only those for this
   // finisher are walked
@@ -2057,9 +2200,9 @@ __device__ __forceinline__ void finish_small_body(const KT* __restrict__ input,
       };
       // from registers when the child fits (read from memory once), else from
       // memory in every pass
-      if (GroupKeys<KT, R>::fits(srcp, c.len)) {  // uniform
-        GroupKeys<KT, R> gk;
-        gk.load(srcp, c.len);
+      if (GroupKeys<KT, R, X>::fits(srcp, c.len)) {  // uniform
+        GroupKeys<KT, R, X> gk;
+        gk.load(srcp, c.len, gk_ext);
         body([&](auto&& f) { gk.for_each(f); });
       } else {
         body([&](auto&& f) { group_for_each(srcp, c.len, f); });
@@ -2077,12 +2220,13 @@ __device__ __forceinline__ void finish_small_body(const KT* __restrict__ input,
 template <typename KT, int C>
 __global__ __launch_bounds__(kFThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_finish_small(
     GGKS_FINISH_PARAMS) {
-  finish_small_body<KT, C, kFRegWords<KT>>(input, red, red2, children, nchildren, cq, cq2, out);
+  finish_small_body<KT, C, kFRegWords<KT>, kFLdsWords>(input, red, red2, children, nchildren, cq, cq2,
+                                                        out);
 }
 template <typename KT, int C>
 __global__ __launch_bounds__(kFThreads) __attribute__((amdgpu_waves_per_eu(6))) void k_finish_small_lite(
     GGKS_FINISH_PARAMS) {
-  finish_small_body<KT, C, kFLiteWords>(input, red, red2, children, nchildren, cq, cq2, out);
+  finish_small_body<KT, C, kFLiteWords, 0>(input, red, red2, children, nchildren, cq, cq2, out);
 }
 #undef GGKS_FINISH_PARAMS
 
@@ -2194,11 +2338,10 @@ __global__ __launch_bounds__(kThreads) void k_finish(const KT* __restrict__ inpu
 
       if (inl && (c.q_cnt > kSortRanks || c.len <= kSortLen)) {
         // many ranks or a short child: sort its keys in LDS, then every rank is a lookup
-        uint32_t np = 1;
-        while (np < c.len) np <<= 1;
+        const uint32_t np = ((uint32_t)c.len + kThreads - 1) / kThreads * kThreads;
         for (uint32_t i = c.len + tid; i < np; i += kThreads) keys[i] = KeyBits<KT>::MAX;
         __syncthreads();
-        lds_bitonic_sort<kThreads, CAP / 2 / kThreads>(keys, np);
+        lds_radix_sort<KT, CAP / kThreads>(keys, np, nbits, hist, misc);
         for (uint32_t i = tid; i < c.q_cnt; i += kThreads) {
           Query q = cq[c.q_off + i];
           out[q.out] = from_key<C>((KT)(lo + keys[q.rank - 1]));

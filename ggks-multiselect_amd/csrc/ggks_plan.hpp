@@ -63,6 +63,9 @@ inline uint32_t radix_budget_log2(uint32_t rb) { return std::max<uint32_t>(rb + 
 // two guard prefixes (keys below / above the sampled digit window).
 inline uint32_t radix_lut_bits(uint32_t rb) { return std::min<uint32_t>(kMaxPrefixBits, rb + 1); }
 
+// Batches of at most this many segments split large segments' LUT builds.
+constexpr size_t kLutSplitSegs = 16;
+
 struct Plan {
   std::vector<SegDesc> segs;
   std::vector<Chunk> chunks;
@@ -74,6 +77,7 @@ struct Plan {
   uint32_t pbits_max = 0;
   // table sizes (equal to the vectors' sizes when the tables are built)
   uint32_t nsegs = 0, nchunks = 0, nitems = 0, nsblocks = 0, ndirect = 0;
+  uint32_t lut_parts = 1;  // most LUT work-groups of a segment (lut_parts())
   // longest direct child, and the largest mean bucket size of an engine
   // segment (a radix segment's whole length): picks the finisher variant
   uint64_t dmax_keys = 0, emax_keys = 0;
@@ -291,6 +295,7 @@ inline void plan_single(Plan& p, int kind, int dtype, uint64_t n, uint32_t nq,
   p.nchunks = (uint32_t)G;
   p.nitems = sh.nitems;
   p.nsblocks = ns / kThreads;
+  p.lut_parts = ggks::lut_parts(sh.mode, ns, (uint32_t)eb);
   p.emax_keys = sh.mode == MODE_RADIX ? n : (n + bmax - 1) / bmax;
 
   p.off_tables = off;
@@ -525,6 +530,11 @@ inline bool plan_batch(Plan& p, int dtype, const std::vector<BatchSeg>& in, cons
   p.nchunks = (uint32_t)p.chunks.size();
   p.nitems = (uint32_t)p.items.size();
   p.nsblocks = (uint32_t)p.sblocks.size();
+  // a batch of many segments builds its LUTs side by side, one work-group
+  // each: split builds would only add work-groups that exit at once
+  if (p.segs.size() <= kLutSplitSegs)
+    for (const SegDesc& sd : p.segs)
+      p.lut_parts = std::max(p.lut_parts, ggks::lut_parts(sd.mode, sd.nsample, (uint32_t)eb));
   p.ndirect = (uint32_t)p.direct_children.size();
   for (const Child& c : p.direct_children) p.dmax_keys = std::max<uint64_t>(p.dmax_keys, c.len);
   for (const SegDesc& sd : p.segs)

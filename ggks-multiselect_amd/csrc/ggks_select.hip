@@ -47,6 +47,18 @@ int fail(int code, const char* fmt, ...) {
   } while (0)
 
 // ------------------------------------------------------------------ context
+// Host staging for the plan tables, k lists and outputs: a ring of pinned
+// buffers, each with an event recorded after the async copies that read it,
+// so a call waits only for the copies issued kPinSlots calls earlier
+// instead of draining the stream (back-to-back async selects with changing
+// tables keep the host ahead of the device).
+constexpr int kPinSlots = 4;
+struct PinSlot {
+  char* buf = nullptr;
+  size_t cap = 0;
+  hipEvent_t ev = nullptr;
+  bool busy = false;  // a copy reading buf may be in flight until ev completes
+};
 }  // namespace ggks
 
 struct ggks_ctx {
@@ -54,8 +66,9 @@ struct ggks_ctx {
   hipStream_t stream = nullptr;
   char* arena = nullptr;
   size_t arena_cap = 0;
-  char* pinned = nullptr;
-  size_t pinned_cap = 0;
+  ggks::PinSlot pin[ggks::kPinSlots];
+  int pin_next = 0;                  // slot the next pin_acquire takes
+  ggks::PinSlot* pin_cur = nullptr;  // slot of the call in progress
   uint32_t* status = nullptr;  // device word
   std::mutex mu;
   // stage timing (ggks_ctx_profile_arm)
@@ -68,7 +81,6 @@ struct ggks_ctx {
   // plan-table cache: the tables in the arena match this signature
   uint64_t table_sig = 0;
   ggks_debug_regions dbg = {};  // the last select's tables (ggks_ctx_debug_regions)
-  bool pinned_busy = false;  // an async copy from `pinned` may be in flight
 };
 
 namespace ggks {
@@ -90,21 +102,50 @@ static int ensure_arena(ggks_ctx* c, size_t bytes) {
   return GGKS_OK;
 }
 
-static int ensure_pinned(ggks_ctx* c, size_t bytes) {
-  if (bytes <= c->pinned_cap) return GGKS_OK;
-  HIP_TRY(hipStreamSynchronize(c->stream));
-  if (c->pinned) HIP_TRY(hipHostFree(c->pinned));
-  c->pinned = nullptr;
-  c->pinned_cap = 0;
-  size_t cap = std::max<size_t>(bytes + bytes / 8, 1 << 16);
-  if (hipHostMalloc(&c->pinned, cap, hipHostMallocDefault) != hipSuccess) {
-    c->pinned = nullptr;
+// Wait until no copy reads the slot, then make it at least `bytes` long.
+static int pin_ready(PinSlot& s, size_t bytes) {
+  if (s.busy) {
+    HIP_TRY(hipEventSynchronize(s.ev));
+    s.busy = false;
+  }
+  // the event only tells the host that the copies reading the buffer are
+  // done, so it needs neither timing nor a system-scope release
+  if (!s.ev) HIP_TRY(hipEventCreateWithFlags(&s.ev, hipEventDisableTiming | hipEventDisableSystemFence));
+  if (bytes <= s.cap) return GGKS_OK;
+  if (s.buf) HIP_TRY(hipHostFree(s.buf));
+  s.buf = nullptr;
+  s.cap = 0;
+  const size_t cap = std::max<size_t>(bytes + bytes / 8, 1 << 16);
+  if (hipHostMalloc(&s.buf, cap, hipHostMallocDefault) != hipSuccess) {
+    s.buf = nullptr;
     (void)hipGetLastError();
     return fail(GGKS_ENOMEM, "cannot allocate %zu bytes of pinned memory", cap);
   }
-  c->pinned_cap = cap;
-  c->pinned_busy = false;
+  s.cap = cap;
   return GGKS_OK;
+}
+
+// The next staging buffer of the ring, at least `bytes` long.
+static int pin_acquire(ggks_ctx* c, size_t bytes, char** out) {
+  PinSlot& s = c->pin[c->pin_next];
+  c->pin_next = (c->pin_next + 1) % kPinSlots;
+  int rc = pin_ready(s, bytes);
+  if (rc) return rc;
+  c->pin_cur = &s;
+  *out = s.buf;
+  return GGKS_OK;
+}
+
+// The async copies reading the acquired buffer are enqueued: mark their end.
+static int pin_mark(ggks_ctx* c) {
+  HIP_TRY(hipEventRecord(c->pin_cur->ev, c->stream));
+  c->pin_cur->busy = true;
+  return GGKS_OK;
+}
+
+// The stream has drained: no staging buffer is read any more.
+static void pin_all_free(ggks_ctx* c) {
+  for (PinSlot& s : c->pin) s.busy = false;
 }
 
 static int ctx_init(ggks_ctx* c, int device, hipStream_t stream) {
@@ -276,10 +317,11 @@ static int enqueue_level(const Exec& x, const Plan& p, const KT* d_src, KT* d_ou
     if (p.nsblocks > 0) {
       const SampleBlock* sblocks = reinterpret_cast<const SampleBlock*>(A + p.off_sblocks);
       hipLaunchKernelGGL((k_sample<KT, C>), dim3(p.nsblocks), dim3(kSampleThreads), 0, s,
-                         d_src, segs, sblocks, samples, minmax, p.seed);
+                         d_src, segs, sblocks, samples, minmax, p.seed, states, lut);
     }
     stage(x, 2);
-    hipLaunchKernelGGL((k_lut<KT>), dim3(nseg), dim3(kThreads), 0, s, segs, samples, minmax, states,
+    hipLaunchKernelGGL((k_lut<KT>), dim3(nseg, p.lut_parts), dim3(kThreads), 0, s, segs, samples,
+                       minmax, states,
                        lut, l2count);
     stage(x, 3);
     if (beside) {  // nseg > 0
@@ -365,16 +407,6 @@ static uint64_t table_signature(const char* p, size_t n, const void* arena) {
   return h;
 }
 
-// The pinned staging buffer is about to be rewritten: wait for the copy that
-// may still be reading it.
-static int pinned_claim(ggks_ctx* c) {
-  if (c->pinned_busy) {
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    c->pinned_busy = false;
-  }
-  return GGKS_OK;
-}
-
 // -------------------------------------------------------------- one select
 enum class KSource { HOST, DEVICE, TOPK };
 
@@ -421,35 +453,34 @@ static int select_one(ggks_ctx* c, int kind, int dtype, const void* d_vec, int64
   const size_t pin_k = align_up(tbytes);
   const size_t pin_out = align_up(pin_k + (ksrc == KSource::HOST ? (size_t)num_k * 4 : 0));
   const size_t pin_total = pin_out + (host_out ? (size_t)num_k * eb : 0);
-  if ((rc = ensure_pinned(c, pin_total))) return rc;
 
   c->prof_cur = c->prof_slots > 0 ? (int)(c->prof_calls++ % (uint64_t)c->prof_slots) : -1;
   mark(c, 0);
   // plan tables: upload only when they differ from what the arena holds
-  {
-    std::vector<char> tab(tbytes);
-    p.write_tables(tab.data());
-    const uint64_t sig = table_signature(tab.data(), tbytes, c->arena);
-    if (sig != c->table_sig) {
-      if ((rc = pinned_claim(c))) return rc;
-      memcpy(c->pinned, tab.data(), tbytes);
-      HIP_TRY(hipMemcpyAsync(c->arena + p.off_tables, c->pinned, tbytes, hipMemcpyHostToDevice,
-                             c->stream));
-      c->pinned_busy = true;
-      c->table_sig = sig;
-    }
+  std::vector<char> tab(tbytes);
+  p.write_tables(tab.data());
+  const uint64_t sig = table_signature(tab.data(), tbytes, c->arena);
+  const bool up_tab = sig != c->table_sig, up_k = ksrc == KSource::HOST;
+  char* pin = nullptr;
+  if (up_tab || up_k || host_out) {
+    if ((rc = pin_acquire(c, pin_total, &pin))) return rc;
+  }
+  if (up_tab) {
+    memcpy(pin, tab.data(), tbytes);
+    HIP_TRY(hipMemcpyAsync(c->arena + p.off_tables, pin, tbytes, hipMemcpyHostToDevice,
+                           c->stream));
+    c->table_sig = sig;
   }
   const uint32_t nk = (uint32_t)num_k;
   const uint32_t qgrid = (nk + 255) / 256;
   const uint32_t* d_k = k;
-  if (ksrc == KSource::HOST) {
-    if ((rc = pinned_claim(c))) return rc;
-    memcpy(c->pinned + pin_k, k, (size_t)num_k * 4);
-    HIP_TRY(hipMemcpyAsync(c->arena + off_dk, c->pinned + pin_k, (size_t)num_k * 4,
+  if (up_k) {
+    memcpy(pin + pin_k, k, (size_t)num_k * 4);
+    HIP_TRY(hipMemcpyAsync(c->arena + off_dk, pin + pin_k, (size_t)num_k * 4,
                            hipMemcpyHostToDevice, c->stream));
-    c->pinned_busy = true;
     d_k = reinterpret_cast<const uint32_t*>(c->arena + off_dk);
   }
+  if ((up_tab || up_k) && (rc = pin_mark(c))) return rc;
   if (p.direct) {  // the direct path takes explicit Query records
     Query* d_q = reinterpret_cast<Query*>(c->arena + p.off_dqueries);
     if (ksrc == KSource::TOPK)
@@ -492,11 +523,11 @@ static int select_one(ggks_ctx* c, int kind, int dtype, const void* d_vec, int64
   if (rc) return rc;
   if (host_out) {
     // pin_out is disjoint from every host-written staging region
-    HIP_TRY(hipMemcpyAsync(c->pinned + pin_out, d_out, (size_t)num_k * eb, hipMemcpyDeviceToHost,
+    HIP_TRY(hipMemcpyAsync(pin + pin_out, d_out, (size_t)num_k * eb, hipMemcpyDeviceToHost,
                            c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
-    c->pinned_busy = false;
-    memcpy(out, c->pinned + pin_out, (size_t)num_k * eb);
+    pin_all_free(c);
+    memcpy(out, pin + pin_out, (size_t)num_k * eb);
   }
   return GGKS_OK;
 }
@@ -541,13 +572,11 @@ static int sort_choose(ggks_ctx* c, int dtype, const void* d_vec, int64_t n, con
   if ((rc = ensure_arena(c, off_out + (size_t)num_k * eb))) return rc;
   c->table_sig = 0;  // the plan tables' place in the arena is overwritten
   const size_t pin_out = align_up((size_t)num_k * 4);
-  if ((rc = ensure_pinned(c, pin_out + (size_t)num_k * eb))) return rc;
-  if ((rc = pinned_claim(c))) return rc;
-  memcpy(c->pinned, k, (size_t)num_k * 4);
+  char* pin = nullptr;
+  if ((rc = pin_acquire(c, pin_out + (size_t)num_k * eb, &pin))) return rc;
+  memcpy(pin, k, (size_t)num_k * 4);
   char* A = c->arena;
-  HIP_TRY(hipMemcpyAsync(A + off_k, c->pinned, (size_t)num_k * 4, hipMemcpyHostToDevice,
-                         c->stream));
-  c->pinned_busy = true;
+  HIP_TRY(hipMemcpyAsync(A + off_k, pin, (size_t)num_k * 4, hipMemcpyHostToDevice, c->stream));
   HIP_TRY(device_sort(dtype, d_vec, A + off_sorted, (uint64_t)n, A + off_temp, tb, c->stream));
   const uint32_t nk = (uint32_t)num_k, grid = (nk + 255) / 256;
   const uint32_t* dk = reinterpret_cast<const uint32_t*>(A + off_k);
@@ -560,11 +589,11 @@ static int sort_choose(ggks_ctx* c, int dtype, const void* d_vec, int64_t n, con
                        reinterpret_cast<const uint32_t*>(A + off_sorted), (uint64_t)n, dk, nk,
                        reinterpret_cast<uint32_t*>(A + off_out));
   HIP_TRY(hipGetLastError());
-  HIP_TRY(hipMemcpyAsync(c->pinned + pin_out, A + off_out, (size_t)num_k * eb,
-                         hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipMemcpyAsync(pin + pin_out, A + off_out, (size_t)num_k * eb, hipMemcpyDeviceToHost,
+                         c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
-  c->pinned_busy = false;
-  memcpy(out, c->pinned + pin_out, (size_t)num_k * eb);
+  pin_all_free(c);
+  memcpy(out, pin + pin_out, (size_t)num_k * eb);
   return GGKS_OK;
 }
 
@@ -721,7 +750,10 @@ int ggks_ctx_destroy(ggks_ctx* c) {
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   if (c->arena) (void)hipFree(c->arena);
-  if (c->pinned) (void)hipHostFree(c->pinned);
+  for (PinSlot& s : c->pin) {
+    if (s.buf) (void)hipHostFree(s.buf);
+    if (s.ev) (void)hipEventDestroy(s.ev);
+  }
   if (c->status) (void)hipFree(c->status);
   delete c;
   return GGKS_OK;
@@ -815,7 +847,10 @@ int ggks_ctx_reserve(ggks_ctx* c, int kind, int dtype, int64_t n, int64_t num_k)
   Plan p;
   ggks_opts o = {};
   plan_single(p, kind, dtype, (uint64_t)n, (uint32_t)std::max<int64_t>(num_k, 1), o);
-  return ensure_pinned(c, p.table_bytes() + 4096);
+  // every staging slot holds the tables and a 1024-entry k list
+  for (PinSlot& s : c->pin)
+    if ((rc = pin_ready(s, p.table_bytes() + 4096))) return rc;
+  return GGKS_OK;
 }
 
 int ggks_ctx_check(ggks_ctx* c) {

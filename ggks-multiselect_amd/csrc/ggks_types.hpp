@@ -94,7 +94,27 @@ struct SegState {         // written on the device by the LUT kernel
   uint32_t hvon;          // 1: the heavy-value bucket is in use
   uint32_t hb;            // F's bucket of hv (L(hv) + 1) when hvon
   uint32_t nfree;         // next free child slot (second-level children)
+  uint32_t arrive;        // LUT parts done (k_lut; zeroed by k_sample)
+  uint32_t pad;
 };
+
+// Work-groups that build one segment's LUT: a sampled segment whose sample
+// is more than kLutPartBytes splits its sample histogram over parts of
+// kLutPartBytes (each histograms its share and adds it into the segment's
+// LUT region, which k_sample zeroes; the last to arrive builds the LUT from
+// the sums), so no single CU reads the whole sample.  At 64 Ki samples the
+// split pays for 8-byte keys (25 -> 20 us) and not for 4-byte keys (12.5 ->
+// 17 us with 16 parts: the parts' device-scope atomics on the same few
+// thousand counters cost more than the reads they spread).
+constexpr uint32_t kLutPartBytes = 128u << 10;
+constexpr uint32_t kLutSplitBytes = 256u << 10;
+#if defined(__HIPCC__)
+__host__ __device__
+#endif
+inline uint32_t lut_parts(uint32_t mode, uint32_t nsample, uint32_t key_bytes) {
+  const uint64_t bytes = (uint64_t)nsample * key_bytes;
+  return (mode == MODE_SAMPLE && bytes > kLutSplitBytes) ? (uint32_t)(bytes / kLutPartBytes) : 1u;
+}
 
 struct Chunk {            // one work-group of the streaming passes
   uint32_t seg;
