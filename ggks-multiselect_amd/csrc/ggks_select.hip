@@ -362,8 +362,10 @@ static int enqueue_level(const Exec& x, const Plan& p, const KT* d_src, KT* d_ou
     if (p.level2) {  // the heavy children's second level (part of the finish stage)
       const uint32_t g2 = std::max<uint32_t>(1, std::min<uint32_t>(p.cmax, kLevel2Groups));
       hipLaunchKernelGGL((k_count2<KT, C>), dim3(g2), dim3(kThreads), 0, s, red, heavy, chunks2,
-                         l2count, p.cmax, rows2, cq, cq2, qb, qr, kids, states, segs, bits2, wb2,
-                         d_out);
+                         l2count, p.cmax, rows2);
+      const uint32_t gr = std::max<uint32_t>(1, std::min<uint32_t>(p.hmax, kLevel2Groups));
+      hipLaunchKernelGGL((k_resolve2<KT, C>), dim3(gr), dim3(kThreads), 0, s, heavy, l2count, p.hmax,
+                         rows2, cq, cq2, qb, qr, kids, states, segs, bits2, wb2, d_out);
       hipLaunchKernelGGL((k_compact2<KT, C>), dim3(g2), dim3(kThreads), 0, s, red, heavy, chunks2,
                          l2count, p.cmax, rows2, bits2, wb2, red2);
     }

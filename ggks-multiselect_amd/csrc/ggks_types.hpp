@@ -159,11 +159,11 @@ struct Child {            // one bucket to finish, or one small whole segment
 // Second level.  A child of more than kHeavyKeys keys found by k_findk is
 // re-bucketed by the whole GPU instead of being finished by one work-group:
 // k_count2 histograms its keys in chunks of kChunk2 keys over kBuckets2
-// buckets (the exact digit (key - lo) >> shift of its key range), the last
-// chunk to finish resolves its ranks (single-key buckets are answered) and
-// lays out its second-level children, and k_compact2 copies their keys into
-// the second reduced buffer at the child's own offsets.  Its children take
-// the heavy child's slot and free slots of its segment's child table.
+// buckets (the exact digit (key - lo) >> shift of its key range),
+// k_resolve2 resolves its ranks (single-key buckets are answered) and lays
+// out its second-level children, and k_compact2 copies their keys into the
+// second reduced buffer at the child's own offsets.  Its children take the
+// heavy child's slot and free slots of its segment's child table.
 constexpr uint64_t kHeavyKeys = 1ull << 16;
 constexpr uint64_t kChunk2 = 1ull << 18;
 constexpr uint32_t kBuckets2 = 4096;
@@ -178,7 +178,7 @@ struct Heavy {             // written by k_findk
   uint32_t chunk_base;     // first second-level chunk
   uint32_t nchunks;
   uint32_t shift;          // bucket = (key - lo) >> shift
-  uint32_t done;           // chunks counted (the last one resolves the ranks)
+  uint32_t done;           // unused (zero)
   uint32_t nact;           // second-level children (k_compact2 skips 0)
   uint32_t slot_base;      // child slots of children 1.. (child 0 takes `slot`)
   uint32_t pad[2];
