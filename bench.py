@@ -493,9 +493,11 @@ def main():
     dom = "count" if stage["count"] >= stage["compact"] else "compact"
     di = L.STAGES.index(dom)
 
-    # the timed region: K steps between two stream synchronisations, with
-    # only the two events around the dominant kernel recorded in each step
-    L.check(lib.ggks_ctx_profile_marks(ctx.handle, (1 << di) | (1 << (di + 1))))
+    # the timed region: K steps between two stream synchronisations; in each
+    # step the dominant kernel's launch carries a start and a stop event
+    # (no marker between the kernels)
+    L.check(lib.ggks_ctx_profile_marks(ctx.handle,
+                                       (1 << di) | (1 << (di + 1)) | L.PROFILE_KERNEL_EVENTS))
     L.check(lib.ggks_ctx_profile_arm(ctx.handle, a.steps))
     L.check(lib.ggks_sync())
     t0 = time.perf_counter()

@@ -47,6 +47,7 @@ GEN_FAMILIES = {"uniform": 0, "normal": 1, "levels": 2, "cauchy": 3, "plateau": 
                 "killer": 6, "noisy": 7, "half_normal": 8, "huge": 9, "ones": 10,
                 "ones_twos": 11, "normal100": 12}
 NSTAGES = 8
+PROFILE_KERNEL_EVENTS = 0x80000000  # GGKS_PROFILE_KERNEL_EVENTS
 NOALLOC_SCRATCH_BYTES = 64 << 20  # GGKS_NOALLOC_SCRATCH_BYTES
 STAGES = ["queries", "sample", "lut", "count", "reduce", "findk", "compact", "finish"]
 

@@ -5,6 +5,7 @@
 // The plan tables go to the device with one async copy from pinned memory,
 // then the level kernels run back to back on the context's stream.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <chrono>
@@ -179,8 +180,38 @@ static int default_ctx(ggks_ctx** out) {
 // a call skips are recorded at the same point, so they measure zero.
 static void mark(ggks_ctx* c, int stage) {
   if (c->prof_cur < 0 || !((c->prof_armed >> stage) & 1u)) return;
+  if (c->prof_armed & GGKS_PROFILE_KERNEL_EVENTS) return;  // carried by the kernel launches
   const int per = GGKS_NSTAGES + 1;
   (void)hipEventRecord(c->prof[(size_t)c->prof_cur * per + stage], c->stream);
+}
+
+// Stages whose kernel launch can carry their events (one kernel each).
+static bool is_kernel_stage(int stage) { return stage == 3 || stage == 6; }
+
+// The events of single-kernel stage `stage` (its own and the next stage's)
+// when they are armed as kernel events, else null: the launch of that
+// stage's kernel then carries them as its start and stop events.
+static void kernel_events(ggks_ctx* c, int stage, hipEvent_t* e0, hipEvent_t* e1) {
+  *e0 = *e1 = nullptr;
+  if (!c || c->prof_cur < 0 || !(c->prof_armed & GGKS_PROFILE_KERNEL_EVENTS)) return;
+  if (!is_kernel_stage(stage)) return;
+  if (!((c->prof_armed >> stage) & 1u) || !((c->prof_armed >> (stage + 1)) & 1u)) return;
+  const int per = GGKS_NSTAGES + 1;
+  *e0 = c->prof[(size_t)c->prof_cur * per + stage];
+  *e1 = c->prof[(size_t)c->prof_cur * per + stage + 1];
+}
+
+// Launch `kernel` on `s`; with stage `stage`'s events armed as kernel
+// events, as their start and stop events.
+template <typename K, typename... Args>
+static void launch_stage(ggks_ctx* prof, int stage, K kernel, dim3 grid, dim3 block, uint32_t lds,
+                         hipStream_t s, Args... args) {
+  hipEvent_t e0, e1;
+  kernel_events(prof, stage, &e0, &e1);
+  if (e0)
+    hipExtLaunchKernelGGL(kernel, grid, block, lds, s, e0, e1, 0, args...);
+  else
+    hipLaunchKernelGGL(kernel, grid, block, lds, s, args...);
 }
 
 // ----------------------------------------------------------- kernel launch
@@ -330,9 +361,9 @@ static int enqueue_level(const Exec& x, const Plan& p, const KT* d_src, KT* d_ou
       finish_direct(x.side);
       HIP_TRY(hipEventRecord(x.join, x.side));
     }
-    hipLaunchKernelGGL((k_count<KT, C>), dim3(nchunks), dim3(kThreads),
-                       ((size_t)p.bmax_max + (1u << p.pbits_max)) * 4, s, d_src, segs, states,
-                       chunks, lut, rows);
+    launch_stage(x.prof, 3, k_count<KT, C>, dim3(nchunks), dim3(kThreads),
+                 (uint32_t)(((size_t)p.bmax_max + (1u << p.pbits_max)) * 4), s, d_src, segs, states,
+                 chunks, lut, rows);
     stage(x, 4);
     hipLaunchKernelGGL(k_reduce, dim3(p.nitems), dim3(kThreads), 0, s, segs,
                        states, items, rows, tot, gsum);
@@ -351,13 +382,13 @@ static int enqueue_level(const Exec& x, const Plan& p, const KT* d_src, KT* d_ou
                          d_out, l2count, heavy, chunks2, p.hmax);
     stage(x, 6);
     if (compact_small(p.pbits_max, p.bmax_max))
-      hipLaunchKernelGGL((k_compact_small<KT, C>), dim3(nchunks), dim3(kThreads),
-                         compact_lds_bytes<true>(), s, d_src, segs, states, chunks, lut, rows,
-                         bits, wbase, kids, red);
+      launch_stage(x.prof, 6, k_compact_small<KT, C>, dim3(nchunks), dim3(kThreads),
+                   (uint32_t)compact_lds_bytes<true>(), s, d_src, segs, states, chunks, lut, rows,
+                   bits, wbase, kids, red);
     else
-      hipLaunchKernelGGL((k_compact<KT, C>), dim3(nchunks), dim3(kThreads),
-                         compact_lds_bytes<false>(), s, d_src, segs, states, chunks, lut, rows,
-                         bits, wbase, kids, red);
+      launch_stage(x.prof, 6, k_compact<KT, C>, dim3(nchunks), dim3(kThreads),
+                   (uint32_t)compact_lds_bytes<false>(), s, d_src, segs, states, chunks, lut, rows,
+                   bits, wbase, kids, red);
     stage(x, 7);
     if (p.level2) {  // the heavy children's second level (part of the finish stage)
       const uint32_t g2 = std::max<uint32_t>(1, std::min<uint32_t>(p.cmax, kLevel2Groups));
@@ -802,7 +833,11 @@ int ggks_ctx_profile_read(ggks_ctx* c, float* ms, int nslots) {
   HIP_TRY(hipStreamSynchronize(c->stream));
   const int per = GGKS_NSTAGES + 1;
   const uint32_t mask = c->prof_armed;
-  auto recorded = [&](int m) { return ((mask >> m) & 1u) != 0; };
+  const bool kev = (mask & GGKS_PROFILE_KERNEL_EVENTS) != 0;
+  auto armed = [&](int m) { return m >= 0 && m <= GGKS_NSTAGES && ((mask >> m) & 1u) != 0; };
+  // with kernel events only the pairs of the kernel-timed stages exist
+  auto kpair = [&](int st) { return is_kernel_stage(st) && armed(st) && armed(st + 1); };
+  auto recorded = [&](int m) { return kev ? (kpair(m) || kpair(m - 1)) : armed(m); };
   int first = -1, last = -1;
   for (int m = 0; m <= GGKS_NSTAGES; ++m)
     if (recorded(m)) {
@@ -828,7 +863,7 @@ int ggks_ctx_profile_read(ggks_ctx* c, float* ms, int nslots) {
 int ggks_ctx_profile_marks(ggks_ctx* c, uint32_t mask) {
   if (!c) return fail(GGKS_EINVAL, "null ctx");
   std::lock_guard<std::mutex> lk(c->mu);
-  c->prof_mask = mask & ((1u << (GGKS_NSTAGES + 1)) - 1u);
+  c->prof_mask = mask & (((1u << (GGKS_NSTAGES + 1)) - 1u) | GGKS_PROFILE_KERNEL_EVENTS);
   return GGKS_OK;
 }
 

@@ -223,7 +223,13 @@ int ggks_ctx_profile_read(ggks_ctx *ctx, float *ms, int nslots);
  * for a recorded event m the time to the next recorded one in ms[.. + m],
  * 0 for the others, and the first to the last recorded event as the whole
  * call.  Takes effect at the next ggks_ctx_profile_arm(), which latches the
- * mask for the selects it times and for their read. */
+ * mask for the selects it times and for their read.
+ * With GGKS_PROFILE_KERNEL_EVENTS set, the events of the single-kernel
+ * stages 3 (count) and 6 (compact) are not recorded between kernels but
+ * passed to that kernel's launch as its start and stop events
+ * (hipExtLaunchKernel), so they add no marker between the kernels and time
+ * the kernel alone; events of the other stages are then not recorded. */
+#define GGKS_PROFILE_KERNEL_EVENTS 0x80000000u
 int ggks_ctx_profile_marks(ggks_ctx *ctx, uint32_t mask);
 
 /* ---- device plumbing for hosts without a GPU framework --------------

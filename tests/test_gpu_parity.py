@@ -202,6 +202,48 @@ def test_stage_events_and_marks(pkg, O):
     L.check(lib.ggks_ctx_profile_marks(ctx.handle, (1 << per) - 1))
 
 
+def test_kernel_events(pkg, O):
+    """Kernel-attached events: with GGKS_PROFILE_KERNEL_EVENTS the count and
+    compaction kernels' launches carry the two events of their stage.  Each
+    stage then reads a positive time no longer than the whole call measured
+    by markers, the other stages read 0, and the outputs are unchanged."""
+    from ggks_multiselect_amd import _lib as L
+    lib = L.lib()
+    W = pkg.workloads
+    n = 1 << 22
+    x = W.normal_f32(7, n)
+    k = klist(W, n, 7, count=64)
+    d = pkg.DeviceArray.from_host(x)
+    dk = pkg.DeviceArray.from_host(k)
+    dout = pkg.DeviceArray(k.size, np.float32)
+    ctx = pkg.Context(0)
+    per = L.NSTAGES + 1
+    want = bits(O.sort_choose(x, k))
+    # the whole call by markers, for the bound
+    L.check(lib.ggks_ctx_profile_marks(ctx.handle, (1 << per) - 1))
+    L.check(lib.ggks_ctx_profile_arm(ctx.handle, 3))
+    for _ in range(3):
+        pkg.multiselect_async(d, dk, dout, ctx)
+    ms = np.zeros(3 * per, np.float32)
+    L.check(lib.ggks_ctx_profile_read(ctx.handle, ms.ctypes.data, 3))
+    whole = float(ms.reshape(3, per)[:, L.NSTAGES].max())
+    for name in ("count", "compact"):
+        si = L.STAGES.index(name)
+        L.check(lib.ggks_ctx_profile_marks(ctx.handle,
+                                           (1 << si) | (1 << (si + 1)) | L.PROFILE_KERNEL_EVENTS))
+        L.check(lib.ggks_ctx_profile_arm(ctx.handle, 3))
+        for _ in range(3):
+            pkg.multiselect_async(d, dk, dout, ctx)
+        ms = np.zeros(3 * per, np.float32)
+        L.check(lib.ggks_ctx_profile_read(ctx.handle, ms.ctypes.data, 3))
+        assert np.array_equal(bits(dout.to_host()), want)
+        for row in ms.reshape(3, per):
+            assert 0 < row[si] <= whole, (name, row[si], whole)
+            assert not np.any(np.delete(row[:L.NSTAGES], si))
+    L.check(lib.ggks_ctx_profile_arm(ctx.handle, 0))
+    L.check(lib.ggks_ctx_profile_marks(ctx.handle, (1 << per) - 1))
+
+
 def test_options_do_not_change_results(pkg, O):
     W = pkg.workloads
     n = (1 << 22) + 1
