@@ -617,6 +617,7 @@ __global__ __launch_bounds__(kSampleThreads) void k_sample(const KT* __restrict_
 // Finisher shapes that k_findk plans for.  k_finish_small takes children of
 // at most kWaveRanks ranks with a work-group of kFThreads.
 constexpr uint32_t kWaveRanks = 4;
+constexpr uint64_t kWaveMaxLen = 1ull << 15;  // longest child k_finish_small takes
 constexpr uint32_t kFThreads = 256;
 // 16-byte words per thread held by k_finish_small: 36 KB of 4-byte keys,
 // 32 KB of 8-byte keys in registers (the most that fit in 128 VGPRs without
@@ -667,6 +668,12 @@ __global__ __launch_bounds__(kThreads) void k_lut(const SegDesc* __restrict__ se
   // every run; k_findk, which counts them, runs after this kernel
   if (l2count && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < 2) l2count[threadIdx.x] = 0;
   __shared__ uint32_t cnt[1u << kMaxPrefixBits];
+  // the sample histogram in four copies, prefix p's in words 4p .. 4p + 3,
+  // a lane counting into copy lane & 3: samples concentrate in a few
+  // prefixes (the dense middle of a normal), and lanes of one instruction
+  // that hit the same prefix then spread over four addresses instead of
+  // serialising on one
+  __shared__ __align__(16) uint32_t cnt4[4u << kMaxPrefixBits];
   __shared__ uint32_t wsum[17];
   __shared__ KT red_lo[16], red_hi[16];
   __shared__ KT sh_hv;
@@ -778,13 +785,23 @@ __global__ __launch_bounds__(kThreads) void k_lut(const SegDesc* __restrict__ se
     if (tid == 0) lut[0] = nb << 16;
     Btot = nb;
   } else {
-    for (uint32_t p = tid; p < P; p += kThreads) cnt[p] = 0;
+    for (uint32_t p = tid; p < P; p += kThreads) reinterpret_cast<uint4*>(cnt4)[p] = make_uint4(0, 0, 0, 0);
     __syncthreads();
+    const uint32_t copy = tid & 3u;
+    // the four copies summed into cnt
+    auto combine = [&]() {
+      for (uint32_t p = tid; p < P; p += kThreads) {
+        const uint4 c4 = reinterpret_cast<const uint4*>(cnt4)[p];
+        cnt[p] = c4.x + c4.y + c4.z + c4.w;
+      }
+    };
     if (nparts == 1) {
       // the whole sample in one round of loads (64 per thread for 4-byte keys)
       strided_for<64 * 4 / sizeof(KT)>(samples + sd.sample_off, sd.nsample, [&](uint32_t, KT k) {
-        atomicAdd(&cnt[(uint32_t)((KT)(k - kb) >> s)], 1u);
+        atomicAdd(&cnt4[((uint32_t)((KT)(k - kb) >> s) << 2) | copy], 1u);
       });
+      __syncthreads();
+      combine();
       __syncthreads();
     } else {
       // this part's share of the sample; the partial counts are added into
@@ -794,8 +811,10 @@ __global__ __launch_bounds__(kThreads) void k_lut(const SegDesc* __restrict__ se
       const uint32_t mine = part + 1 < nparts ? per : sd.nsample - part * per;
       strided_for<kLutPartBytes / sizeof(KT) / kThreads>(samples + sd.sample_off + part * per, mine,
                                               [&](uint32_t, KT k) {
-        atomicAdd(&cnt[(uint32_t)((KT)(k - kb) >> s)], 1u);
+        atomicAdd(&cnt4[((uint32_t)((KT)(k - kb) >> s) << 2) | copy], 1u);
       });
+      __syncthreads();
+      combine();
       __syncthreads();
       for (uint32_t p = tid; p < P; p += kThreads)
         if (cnt[p]) atomicAdd(&lut[p], cnt[p]);
@@ -1220,6 +1239,8 @@ __device__ __forceinline__ void findk_segment(const SegDesc* __restrict__ segs,
   __syncthreads();
 
   // C: group ranks by child (hist is free now: counters, then offsets)
+  __shared__ uint32_t lsplit;  // child slots taken past U by split children
+  if (tid == 0) lsplit = 0;
   for (uint32_t u = tid; u < U; u += kThreads) hist[u] = 0;
   __syncthreads();
   for (uint32_t i = tid; i < sd.q_cnt; i += kThreads) {
@@ -1251,6 +1272,21 @@ __device__ __forceinline__ void findk_segment(const SegDesc* __restrict__ segs,
       cp->lo = klo;
       cp->hi = khi;
       cp->need_range = 0;
+    }
+    // a child of 2..kWaveRanks ranks for the 256-thread finisher (and no
+    // second level, which takes the free slots) is split into one-rank
+    // records, the extra ones in free child slots: its ranks are then
+    // finished side by side by as many work-groups, not one after another
+    if (!l2 && next - first > 1 && next - first <= kWaveRanks && clen <= kWaveMaxLen) {
+      const uint32_t extra = next - first - 1;
+      const uint32_t slot = U + atomicAdd(&lsplit, extra);
+      Child one = *cp;
+      one.q_cnt = 1;
+      for (uint32_t j = 0; j < extra; ++j) {
+        one.q_off = sd.q_off + first + 1 + j;
+        children[sd.q_off + slot + j] = one;
+      }
+      cp->q_cnt = 1;
     }
     if (l2 && clen > kHeavyKeys) {
       // a heavy child goes to the second level: a heavy-child record and its
@@ -1303,7 +1339,8 @@ __device__ __forceinline__ void findk_segment(const SegDesc* __restrict__ segs,
     cq[sd.q_off + pos] = q;
   }
   // unused child slots finish nothing
-  for (uint32_t u = U + tid; u < sd.q_cnt; u += kThreads) {
+  __syncthreads();  // lsplit is final
+  for (uint32_t u = U + lsplit + tid; u < sd.q_cnt; u += kThreads) {
     children[sd.q_off + u].q_cnt = 0;
     children[sd.q_off + u].len = 0;
   }
@@ -1828,7 +1865,6 @@ __global__ __launch_bounds__(kThreads) void k_copy16(const uint4* __restrict__ s
 // Children with few ranks are finished by k_finish_small (one 256-thread
 // work-group each); the others by k_finish (a 1024-thread work-group with
 // the child in LDS).
-constexpr uint64_t kWaveMaxLen = 1ull << 15;
 __device__ __forceinline__ bool wave_child(const Child& c) {
   return c.q_cnt <= kWaveRanks && c.len <= kWaveMaxLen;
 }

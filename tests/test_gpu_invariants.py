@@ -16,7 +16,8 @@ fixes, not against a second copy of the kernels' arithmetic:
   bucket's keys;
 - each child query's rank r satisfies (keys below the bucket) + r = the
   requested order statistic's ascending rank n - k + 1, and the answers
-  equal the oracle's.
+  equal the oracle's; a child of a few ranks may be split into one-rank
+  copies in the free child slots, each with its bucket's slice.
 
 The second level is off (level2 = -1) so the child table stays as the
 first level wrote it.  Cases cover the sample and radix bucket functions,
@@ -132,7 +133,17 @@ def test_select_invariants(pkg, O, case):
     kvals = np.asarray(k, np.int64)
     cq_all = read(lib, g.cq, QUERY, int(sd["q_off"]) + len(k))
     seen = []
-    for c in kids:
+    # slots past the U children: one-rank copies of split children (same
+    # bucket and slice, one of its ranks each), the rest unused (no ranks)
+    extra = read(lib, g.children + CHILD.itemsize * (int(sd["q_off"]) + U), CHILD, len(k) - U)
+    byb = {int(c["bucket"]): c for c in kids}
+    for c in extra:
+        if int(c["q_cnt"]) == 0:
+            continue
+        assert int(c["q_cnt"]) == 1
+        own = byb[int(c["bucket"])]
+        assert int(c["start"]) == int(own["start"]) and int(c["len"]) == int(own["len"])
+    for c in list(kids) + [c for c in extra if int(c["q_cnt"]) > 0]:
         bk = int(c["bucket"])
         sel = kby[below[bk]:below[bk + 1]]
         if not int(c["need_range"]):  # a planned key range holds the bucket's keys
