@@ -618,6 +618,25 @@ __global__ __launch_bounds__(kSampleThreads) void k_sample(const KT* __restrict_
 // at most kWaveRanks ranks with a work-group of kFThreads.
 constexpr uint32_t kWaveRanks = 4;
 constexpr uint64_t kWaveMaxLen = 1ull << 15;  // longest child k_finish_small takes
+// Children with few ranks are finished by k_finish_small (one 256-thread
+// work-group each); the others by k_finish (a 1024-thread work-group with
+// the child in LDS).
+__host__ __device__ __forceinline__ bool wave_child(uint64_t len, uint32_t q_cnt) {
+  return q_cnt <= kWaveRanks && len <= kWaveMaxLen;
+}
+__device__ __forceinline__ bool wave_child(const Child& c) { return wave_child(c.len, c.q_cnt); }
+// Children with more than kWaveRanks ranks and at most kSortCap keys (top-K
+// buckets, mostly) are sorted by k_finish_small too; k_finish takes what
+// fits neither.
+constexpr uint32_t kSortCap = 1024;
+__host__ __device__ __forceinline__ bool sort_child(uint64_t len, uint32_t q_cnt) {
+  return !wave_child(len, q_cnt) && len <= kSortCap;
+}
+__device__ __forceinline__ bool sort_child(const Child& c) { return sort_child(c.len, c.q_cnt); }
+// A child with ranks that neither fits: k_finish's.
+__host__ __device__ __forceinline__ bool big_child(uint64_t len, uint32_t q_cnt) {
+  return q_cnt != 0 && !wave_child(len, q_cnt) && !sort_child(len, q_cnt);
+}
 constexpr uint32_t kFThreads = 256;
 // 16-byte words per thread held by k_finish_small: 36 KB of 4-byte keys,
 // 32 KB of 8-byte keys in registers (the most that fit in 128 VGPRs without
@@ -663,10 +682,13 @@ __global__ __launch_bounds__(kThreads) void k_lut(const SegDesc* __restrict__ se
                                                   const KT* __restrict__ minmax,
                                                   SegState* __restrict__ states,
                                                   uint32_t* __restrict__ lut_pool,
-                                                  uint32_t* __restrict__ l2count) {
+                                                  uint32_t* __restrict__ l2count,
+                                                  uint32_t* __restrict__ bigw) {
   // the second level's counters (heavy children, chunks) start at zero in
   // every run; k_findk, which counts them, runs after this kernel
   if (l2count && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < 2) l2count[threadIdx.x] = 0;
+  // k_findk counts the children k_finish takes here
+  if (bigw && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *bigw = 0;
   __shared__ uint32_t cnt[1u << kMaxPrefixBits];
   // the sample histogram in four copies, prefix p's in words 4p .. 4p + 3,
   // a lane counting into copy lane & 3: samples concentrate in a few
@@ -1103,7 +1125,9 @@ __device__ __forceinline__ void findk_segment(const SegDesc* __restrict__ segs,
                                               KT* __restrict__ out,
                                               uint32_t* __restrict__ l2count,
                                               Heavy* __restrict__ heavy,
-                                              Chunk2* __restrict__ chunks2, uint32_t hmax) {
+                                              Chunk2* __restrict__ chunks2, uint32_t hmax,
+                                              uint32_t* __restrict__ bigw,
+                                              uint2* __restrict__ cinfo) {
   extern __shared__ __align__(16) uint32_t lds[];
   const SegDesc sd = segs[blockIdx.x];
   // the first k of this thread, read now so its latency hides under the
@@ -1235,12 +1259,17 @@ __device__ __forceinline__ void findk_segment(const SegDesc* __restrict__ segs,
     c.bucket = b;
     c.seg = blockIdx.x;
     children[sd.q_off + u] = c;
+    cinfo[sd.q_off + u] = make_uint2(start, b);
   }
   __syncthreads();
 
   // C: group ranks by child (hist is free now: counters, then offsets)
   __shared__ uint32_t lsplit;  // child slots taken past U by split children
-  if (tid == 0) lsplit = 0;
+  __shared__ uint32_t lbig;    // children for k_finish
+  if (tid == 0) {
+    lsplit = 0;
+    lbig = 0;
+  }
   for (uint32_t u = tid; u < U; u += kThreads) hist[u] = 0;
   __syncthreads();
   for (uint32_t i = tid; i < sd.q_cnt; i += kThreads) {
@@ -1326,6 +1355,7 @@ __device__ __forceinline__ void findk_segment(const SegDesc* __restrict__ segs,
         cp->q_cnt = 0;
       }
     }
+    if (big_child(cp->len, cp->q_cnt)) atomicAdd(&lbig, 1u);
   }
   if (l2 && tid == 0) states[blockIdx.x].nfree = U;  // first free child slot
   __syncthreads();
@@ -1344,7 +1374,10 @@ __device__ __forceinline__ void findk_segment(const SegDesc* __restrict__ segs,
     children[sd.q_off + u].q_cnt = 0;
     children[sd.q_off + u].len = 0;
   }
-  if (tid == 0) states[blockIdx.x].nact = U;
+  if (tid == 0) {
+    states[blockIdx.x].nact = U;
+    if (bigw && lbig) atomicAdd(bigw, lbig);
+  }
 }
 
 #define GGKS_FINDK_PARAMS                                                                      \
@@ -1355,10 +1388,10 @@ __device__ __forceinline__ void findk_segment(const SegDesc* __restrict__ segs,
       uint32_t *__restrict__ qr, Child *__restrict__ children, Query *__restrict__ cq,         \
       uint32_t *__restrict__ bits_pool, uint32_t *__restrict__ wbase_pool, KT *__restrict__ out, \
       uint32_t *__restrict__ l2count, Heavy *__restrict__ heavy, Chunk2 *__restrict__ chunks2,  \
-      uint32_t hmax
+      uint32_t hmax, uint32_t *__restrict__ bigw, uint2 *__restrict__ cinfo
 #define GGKS_FINDK_ARGS                                                                        \
   segs, states, lut_pool, totals, gsums, queries, kraw, status, qb, qr, children, cq, bits_pool, \
-      wbase_pool, out, l2count, heavy, chunks2, hmax
+      wbase_pool, out, l2count, heavy, chunks2, hmax, bigw, cinfo
 constexpr uint32_t kFindkSmallBuckets = 4 * kThreads;  // one 16-byte word per thread
 template <typename KT, int C>
 __global__ __launch_bounds__(kThreads) void k_findk(GGKS_FINDK_PARAMS) {
@@ -1417,7 +1450,7 @@ __device__ __forceinline__ void compact_chunk(const KT* __restrict__ src,
                                               uint32_t* __restrict__ rows,
                                               const uint32_t* __restrict__ bits_pool,
                                               const uint32_t* __restrict__ wbase_pool,
-                                              const Child* __restrict__ children,
+                                              const uint2* __restrict__ cinfo,
                                               KT* __restrict__ red) {
   using Cfg = CompactCfg<SMALL>;
   constexpr int kRound = Cfg::round;
@@ -1461,14 +1494,14 @@ __device__ __forceinline__ void compact_chunk(const KT* __restrict__ src,
   }
   const uint8_t* const act_of = reinterpret_cast<const uint8_t*>(abyte);
   uint32_t* row = rows + sd.row_base + (uint64_t)ch.idx * sd.bmax;
-  const Child* kids = children + sd.q_off;
+  const uint2* kinfo = cinfo + sd.q_off;  // per child: slice start in the segment's part, bucket
   // Cursors are positions inside the segment's part of the reduced buffer
   // (child start - red_base < segment length < 2^32), so they stay 32-bit
   // however far into the buffer the segment sits (batches and served groups
   // of more than 2^32 keys).
   KT* const rseg = red + sd.red_base;
   for (uint32_t u = tid; u < U && u < kCompactCursors; u += kThreads)
-    cur[u] = (uint32_t)(kids[u].start - sd.red_base) + row[kids[u].bucket];
+    cur[u] = kinfo[u].x + row[kinfo[u].y];
   if (tid == 0) nbuf = 0;
   __syncthreads();
   BucketFn<KT> bk;
@@ -1482,8 +1515,7 @@ __device__ __forceinline__ void compact_chunk(const KT* __restrict__ src,
       const uint32_t u = wbase[b >> 5] + __popc(bits & ((1u << sh) - 1u));
       const uint32_t pos = u < kCompactCursors
                                ? atomicAdd(&cur[u], 1u)
-                               : (uint32_t)(kids[u].start - sd.red_base) +
-                                     atomicAdd(&row[kids[u].bucket], 1u);
+                               : kinfo[u].x + atomicAdd(&row[kinfo[u].y], 1u);
       rseg[pos] = raw;
     }
   };
@@ -1507,8 +1539,7 @@ __device__ __forceinline__ void compact_chunk(const KT* __restrict__ src,
       for (int f = 0; f < F; ++f) {
         const uint32_t pos = u[f] < kCompactCursors
                                  ? atomicAdd(&cur[u[f]], 1u)
-                                 : (uint32_t)(kids[u[f]].start - sd.red_base) +
-                                       atomicAdd(&row[kids[u[f]].bucket], 1u);
+                                 : kinfo[u[f]].x + atomicAdd(&row[kinfo[u[f]].y], 1u);
         rseg[pos] = kk[f];
       }
     }
@@ -1604,18 +1635,18 @@ __device__ __forceinline__ void compact_chunk(const KT* __restrict__ src,
       const SegState *__restrict__ states, const Chunk *__restrict__ chunks,                  \
       const uint32_t *__restrict__ lut_pool, uint32_t *__restrict__ rows,                     \
       const uint32_t *__restrict__ bits_pool, const uint32_t *__restrict__ wbase_pool,        \
-      const Child *__restrict__ children, KT *__restrict__ red
+      const uint2 *__restrict__ cinfo, KT *__restrict__ red
 template <typename KT, int C>
 __global__ __launch_bounds__(kThreads) void k_compact(GGKS_COMPACT_PARAMS) {
   compact_chunk<KT, C, false>(src, segs, states, chunks, lut_pool, rows, bits_pool, wbase_pool,
-                              children, red);
+                              cinfo, red);
 }
 // at most 64 VGPRs: two 1024-thread work-groups per CU
 template <typename KT, int C>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) void k_compact_small(
     GGKS_COMPACT_PARAMS) {
   compact_chunk<KT, C, true>(src, segs, states, chunks, lut_pool, rows, bits_pool, wbase_pool,
-                             children, red);
+                             cinfo, red);
 }
 #undef GGKS_COMPACT_PARAMS
 
@@ -1862,19 +1893,6 @@ __global__ __launch_bounds__(kThreads) void k_copy16(const uint4* __restrict__ s
     dst[i] = stream_load(&src[i]);
 }
 
-// Children with few ranks are finished by k_finish_small (one 256-thread
-// work-group each); the others by k_finish (a 1024-thread work-group with
-// the child in LDS).
-__device__ __forceinline__ bool wave_child(const Child& c) {
-  return c.q_cnt <= kWaveRanks && c.len <= kWaveMaxLen;
-}
-// Children with more than kWaveRanks ranks and at most kSortCap keys (top-K
-// buckets, mostly) are sorted by k_finish_small too; k_finish takes what
-// fits neither.
-constexpr uint32_t kSortCap = 1024;
-__device__ __forceinline__ bool sort_child(const Child& c) {
-  return !wave_child(c) && c.len <= kSortCap;
-}
 // k_finish sorts a child held in LDS when it has many ranks or is short
 // enough that a sort beats the digit passes.
 constexpr uint32_t kSortLen = 4096;
@@ -2294,7 +2312,10 @@ __global__ __launch_bounds__(kThreads) void k_finish(const KT* __restrict__ inpu
                                                      uint32_t nchildren,
                                                      const Query* __restrict__ cq0,
                                                      const Query* __restrict__ cq2,
-                                                     KT* __restrict__ out) {
+                                                     KT* __restrict__ out,
+                                                     const uint32_t* __restrict__ bigw) {
+  // nothing to do when k_findk counted no child for this finisher
+  if (bigw && *bigw == 0) return;  // uniform
   constexpr uint32_t W = KeyBits<KT>::W;
   constexpr uint32_t CAP = kFinishLdsBytes / sizeof(KT);
   extern __shared__ __align__(16) uint32_t lds[];

@@ -92,6 +92,8 @@ struct Plan {
   size_t off_states = 0, off_samples = 0, off_minmax = 0, off_lut = 0, off_rows = 0, off_tot = 0;
   size_t off_bits = 0, off_wbase = 0, off_q = 0, off_qb = 0, off_qr = 0;
   size_t off_gsum = 0;  // per-group bucket sums (k_reduce -> k_findk)
+  size_t off_big = 0;   // children k_finish takes (k_findk counts, k_finish reads)
+  size_t off_cinfo = 0;  // per child slot: slice start in its segment's part, bucket
   size_t off_children = 0, off_cq = 0, off_red = 0, total = 0;
   size_t off_kvals = 0, off_dchildren = 0, off_dqueries = 0;  // batch / direct tables
   std::vector<uint32_t> kvals;         // k lists of the engine segments, by q_off
@@ -317,8 +319,10 @@ inline void plan_single(Plan& p, int kind, int dtype, uint64_t n, uint32_t nq,
   p.off_qb = take((size_t)nq * 4);
   p.off_qr = take((size_t)nq * 4);
   p.off_children = take((size_t)nq * sizeof(Child));
+  p.off_cinfo = take((size_t)nq * 8);
   p.off_cq = take((size_t)nq * sizeof(Query));
   p.off_red = take((size_t)n * eb);
+  p.off_big = take(4);
   layout_level2(p, take, n, nq, eb);
   p.total = off;
 }
@@ -569,8 +573,10 @@ inline bool plan_batch(Plan& p, int dtype, const std::vector<BatchSeg>& in, cons
   p.off_qb = take((size_t)qoff * 4);
   p.off_qr = take((size_t)qoff * 4);
   p.off_children = take((size_t)qoff * sizeof(Child));
+  p.off_cinfo = take((size_t)qoff * 8);
   p.off_cq = take((size_t)qoff * sizeof(Query));
   p.off_red = take((size_t)red * eb);
+  p.off_big = take(4);
   layout_level2(p, take, red, qoff, eb);
   p.total = off;
   return true;

@@ -267,7 +267,7 @@ static void level_attrs(const Plan&) {
 template <typename KT, int C>
 static void finish(hipStream_t s, const KT* d_src, const KT* red, const KT* red2,
                    const Child* kids, uint32_t n, const Query* cq, const Query* cq2, KT* d_out,
-                   uint32_t max_groups, bool lite) {
+                   uint32_t max_groups, bool lite, bool big, const uint32_t* bigw) {
   bool launched = false;
   if constexpr (sizeof(KT) == 4) {
     if (lite) {
@@ -279,8 +279,11 @@ static void finish(hipStream_t s, const KT* d_src, const KT* red, const KT* red2
   if (!launched)
     hipLaunchKernelGGL((k_finish_small<KT, C>), dim3(std::min<uint32_t>(n, 65536)),
                        dim3(kFThreads), 0, s, d_src, red, red2, kids, n, cq, cq2, d_out);
-  hipLaunchKernelGGL((k_finish<KT, C>), dim3(std::min<uint32_t>(n, max_groups)), dim3(kThreads),
-                     kFinishLdsTotal, s, d_src, red, red2, kids, n, cq, cq2, d_out);
+  // big = false: no child for k_finish (known on the host); bigw: a device
+  // count of them, read by k_finish before anything else
+  if (big)
+    hipLaunchKernelGGL((k_finish<KT, C>), dim3(std::min<uint32_t>(n, max_groups)), dim3(kThreads),
+                       kFinishLdsTotal, s, d_src, red, red2, kids, n, cq, cq2, d_out, bigw);
 }
 
 // Grid of the second-level kernels (they walk the chunk table with a grid
@@ -307,11 +310,15 @@ static int enqueue_level(const Exec& x, const Plan& p, const KT* d_src, KT* d_ou
   // an engine segment's by its mean bucket size
   const uint64_t lite_keys = finish_small_reg_keys<KT>(true);
   const bool dlite = p.dmax_keys <= lite_keys, elite = 2 * p.emax_keys <= lite_keys;
+  // whether a direct child (planned on the host) is one for k_finish; a plan
+  // without its host tables (the no-allocation path) may have one
+  bool dbig = ndirect > 0 && p.direct_children.empty();
+  for (const Child& c : p.direct_children) dbig = dbig || big_child(c.len, c.q_cnt);
   auto finish_direct = [&](hipStream_t ds) {
     const Child* dk = reinterpret_cast<const Child*>(A + p.off_dchildren);
     const Query* dq = reinterpret_cast<const Query*>(A + p.off_dqueries);
     finish<KT, C>(ds, d_src, (const KT*)nullptr, (const KT*)nullptr, dk, ndirect, dq,
-                  (const Query*)nullptr, d_out, 256, dlite);
+                  (const Query*)nullptr, d_out, 256, dlite, dbig, (const uint32_t*)nullptr);
   };
   const uint32_t nseg = p.nsegs;
   if (nseg > 0) {
@@ -351,9 +358,11 @@ static int enqueue_level(const Exec& x, const Plan& p, const KT* d_src, KT* d_ou
                          d_src, segs, sblocks, samples, minmax, p.seed, states, lut);
     }
     stage(x, 2);
+    uint32_t* bigw = reinterpret_cast<uint32_t*>(A + p.off_big);
+    uint2* cinfo = reinterpret_cast<uint2*>(A + p.off_cinfo);
     hipLaunchKernelGGL((k_lut<KT>), dim3(nseg, p.lut_parts), dim3(kThreads), 0, s, segs, samples,
                        minmax, states,
-                       lut, l2count);
+                       lut, l2count, bigw);
     stage(x, 3);
     if (beside) {  // nseg > 0
       HIP_TRY(hipEventRecord(x.fork, s));
@@ -373,22 +382,22 @@ static int enqueue_level(const Exec& x, const Plan& p, const KT* d_src, KT* d_ou
                          ((size_t)p.bmax_max + (1u << p.pbits_max) + 3 * ((p.bmax_max + 31) / 32) +
                           32) * 4,
                          s, segs, states, lut, tot, gsum, q, d_k, x.status, qb, qr, kids, cq, bits, wbase,
-                         d_out, l2count, heavy, chunks2, p.hmax);
+                         d_out, l2count, heavy, chunks2, p.hmax, bigw, cinfo);
     else
       hipLaunchKernelGGL((k_findk<KT, C>), dim3(nseg), dim3(kThreads),
                          ((size_t)p.bmax_max + (1u << p.pbits_max) + 3 * ((p.bmax_max + 31) / 32) +
                           32) * 4,
                          s, segs, states, lut, tot, gsum, q, d_k, x.status, qb, qr, kids, cq, bits, wbase,
-                         d_out, l2count, heavy, chunks2, p.hmax);
+                         d_out, l2count, heavy, chunks2, p.hmax, bigw, cinfo);
     stage(x, 6);
     if (compact_small(p.pbits_max, p.bmax_max))
       launch_stage(x.prof, 6, k_compact_small<KT, C>, dim3(nchunks), dim3(kThreads),
                    (uint32_t)compact_lds_bytes<true>(), s, d_src, segs, states, chunks, lut, rows,
-                   bits, wbase, kids, red);
+                   bits, wbase, cinfo, red);
     else
       launch_stage(x.prof, 6, k_compact<KT, C>, dim3(nchunks), dim3(kThreads),
                    (uint32_t)compact_lds_bytes<false>(), s, d_src, segs, states, chunks, lut, rows,
-                   bits, wbase, kids, red);
+                   bits, wbase, cinfo, red);
     stage(x, 7);
     if (p.level2) {  // the heavy children's second level (part of the finish stage)
       const uint32_t g2 = std::max<uint32_t>(1, std::min<uint32_t>(p.cmax, kLevel2Groups));
@@ -400,7 +409,9 @@ static int enqueue_level(const Exec& x, const Plan& p, const KT* d_src, KT* d_ou
       hipLaunchKernelGGL((k_compact2<KT, C>), dim3(g2), dim3(kThreads), 0, s, red, heavy, chunks2,
                          l2count, p.cmax, rows2, bits2, wb2, red2);
     }
-    finish<KT, C>(s, d_src, red, red2, kids, p.nq, cq, cq2, d_out, 256, elite);
+    // the second level's children are not counted: k_finish screens them
+    finish<KT, C>(s, d_src, red, red2, kids, p.nq, cq, cq2, d_out, 256, elite, true,
+                  p.level2 ? (const uint32_t*)nullptr : bigw);
   } else {
     for (int st = 1; st < GGKS_NSTAGES; ++st) stage(x, st);
   }
