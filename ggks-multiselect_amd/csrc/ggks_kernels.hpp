@@ -538,8 +538,10 @@ __global__ __launch_bounds__(kSampleThreads) void k_sample(const KT* __restrict_
                                                            KT* __restrict__ samples,
                                                            KT* __restrict__ minmax, uint64_t seed,
                                                            SegState* __restrict__ states,
-                                                           uint32_t* __restrict__ lut_pool) {
+                                                           uint32_t* __restrict__ lut_pool,
+                                                           uint32_t* __restrict__ hgram) {
   __shared__ KT red_lo[kSampleWaves], red_hi[kSampleWaves], cand[kSampleWaves];
+  __shared__ uint32_t hl[1u << kMaxPrefixBits];  // the block's keys by their top pbits bits
   __shared__ uint32_t ccnt[kSampleWaves][kSampleWaves];  // [wave][candidate]
   const SampleBlock sb = sblocks[blockIdx.x];
   const SegDesc& sd = segs[sb.seg];
@@ -569,6 +571,19 @@ __global__ __launch_bounds__(kSampleThreads) void k_sample(const KT* __restrict_
     samples[sd.sample_off + sb.blk * kThreads + j * kSampleThreads + threadIdx.x] = x;
     lo = x < lo ? x : lo;
     hi = x > hi ? x : hi;
+  }
+  // With a histogram (one sampled segment) the sample is also counted by
+  // the top pbits bits of the key, into hgram: where the sampled range spans
+  // the top bit (full-width ranges, such as floats of both signs) the LUT's
+  // prefixes are exactly these bins, and k_lut reads 2^pbits counts instead
+  // of histogramming the whole sample on one CU.
+  const bool hg = hgram != nullptr && sd.mode == MODE_SAMPLE;  // uniform
+  const uint32_t hsh = KeyBits<KT>::W - sd.pbits;
+  if (hg) {
+    for (uint32_t i = threadIdx.x; i < (1u << sd.pbits); i += kSampleThreads) hl[i] = 0;
+    __syncthreads();
+#pragma unroll
+    for (uint32_t j = 0; j < kSamplePer; ++j) atomicAdd(&hl[(uint32_t)(k[j] >> hsh)], 1u);
   }
   if (lane == 0) cand[w] = k[0];
   for (int d = 32; d >= 1; d >>= 1) {
@@ -611,6 +626,10 @@ __global__ __launch_bounds__(kSampleThreads) void k_sample(const KT* __restrict_
     mm[1] = hi;
     mm[2] = best;
     mm[3] = (KT)bn;
+  }
+  if (hg) {  // hl is complete: the barrier above follows every count
+    for (uint32_t i = threadIdx.x; i < (1u << sd.pbits); i += kSampleThreads)
+      if (hl[i]) atomicAdd(&hgram[i], hl[i]);
   }
 }
 
@@ -683,7 +702,8 @@ __global__ __launch_bounds__(kThreads) void k_lut(const SegDesc* __restrict__ se
                                                   SegState* __restrict__ states,
                                                   uint32_t* __restrict__ lut_pool,
                                                   uint32_t* __restrict__ l2count,
-                                                  uint32_t* __restrict__ bigw) {
+                                                  uint32_t* __restrict__ bigw,
+                                                  uint32_t* __restrict__ hgram) {
   // the second level's counters (heavy children, chunks) start at zero in
   // every run; k_findk, which counts them, runs after this kernel
   if (l2count && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < 2) l2count[threadIdx.x] = 0;
@@ -776,6 +796,13 @@ __global__ __launch_bounds__(kThreads) void k_lut(const SegDesc* __restrict__ se
     pb = 0;
     s = (uint32_t)bl;
     P = 1;
+  } else if (hgram && bl == (int)W) {
+    // full-width range with k_sample's histogram: prefixes aligned to
+    // multiples of 2^s, so the prefix of a key is (key >> s) - (kb >> s)
+    // and its count is hgram[key >> s]
+    s = W - pb;
+    kb = (KT)((kmin >> s) << s);
+    P = (uint32_t)((kmax >> s) - (kb >> s)) + 1;  // <= 2^pb
   } else {
     s = bl > (int)pb ? (uint32_t)(bl - (int)pb) : 0u;
     P = (uint32_t)(((uint64_t)R >> s) + 1);  // <= 2^pb
@@ -807,6 +834,8 @@ __global__ __launch_bounds__(kThreads) void k_lut(const SegDesc* __restrict__ se
     if (tid == 0) lut[0] = nb << 16;
     Btot = nb;
   } else {
+    const bool useh = hgram && bl == (int)W;  // uniform
+    if (useh && part != 0) return;  // uniform: one work-group reads the counts
     for (uint32_t p = tid; p < P; p += kThreads) reinterpret_cast<uint4*>(cnt4)[p] = make_uint4(0, 0, 0, 0);
     __syncthreads();
     const uint32_t copy = tid & 3u;
@@ -817,7 +846,11 @@ __global__ __launch_bounds__(kThreads) void k_lut(const SegDesc* __restrict__ se
         cnt[p] = c4.x + c4.y + c4.z + c4.w;
       }
     };
-    if (nparts == 1) {
+    if (useh) {
+      const uint32_t h0 = (uint32_t)(kb >> s);
+      for (uint32_t p = tid; p < P; p += kThreads) cnt[p] = hgram[h0 + p];
+      __syncthreads();
+    } else if (nparts == 1) {
       // the whole sample in one round of loads (64 per thread for 4-byte keys)
       strided_for<64 * 4 / sizeof(KT)>(samples + sd.sample_off, sd.nsample, [&](uint32_t, KT k) {
         atomicAdd(&cnt4[((uint32_t)((KT)(k - kb) >> s) << 2) | copy], 1u);
@@ -850,16 +883,20 @@ __global__ __launch_bounds__(kThreads) void k_lut(const SegDesc* __restrict__ se
       __syncthreads();
     }
     // prefixes holding no sample: a run of them shares one bucket
-    uint32_t nonempty = 0, runs = 0;
+    uint32_t nonempty = 0, runs = 0, csum = 0;
     for (uint32_t p = tid; p < P; p += kThreads) {
+      csum += cnt[p];
       if (cnt[p]) nonempty++;
       else if (p == 0 || cnt[p - 1]) runs++;
     }
-    uint32_t t1, t2;
+    uint32_t t1, t2, t3;
     block_excl_scan(nonempty, wsum, t1);
     block_excl_scan(runs, wsum, t2);
+    block_excl_scan(csum, wsum, t3);
     const uint32_t A = Bt > t1 + t2 ? Bt - t1 - t2 : 0u;
-    const uint32_t ns = sd.nsample;
+    // the counts' own total (the sample size; the sub-bucket shares below
+    // stay within the budget whatever the counts sum to)
+    const uint32_t ns = t3 > 0 ? t3 : 1u;
     const uint32_t per = (P + kThreads - 1) / kThreads;
     const uint32_t p0 = tid * per, p1 = min(P, p0 + per);
     auto width = [&](uint32_t p, uint32_t& nb) -> uint32_t {
@@ -911,6 +948,10 @@ __global__ __launch_bounds__(kThreads) void k_lut(const SegDesc* __restrict__ se
     }
     states[blockIdx.x] = st;
   }
+  // k_sample counts into a zeroed histogram: clear it for the next select
+  // (every read of it is behind the barriers above)
+  if (hgram && sd.mode == MODE_SAMPLE)
+    for (uint32_t i = tid; i < (1u << sd.pbits); i += kThreads) hgram[i] = 0;
 }
 
 // Pass 1: per-chunk LDS histogram of bucket ids.

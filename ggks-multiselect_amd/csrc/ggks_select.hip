@@ -71,6 +71,7 @@ struct ggks_ctx {
   int pin_next = 0;                  // slot the next pin_acquire takes
   ggks::PinSlot* pin_cur = nullptr;  // slot of the call in progress
   uint32_t* status = nullptr;  // device word
+  uint32_t* hgram = nullptr;   // sample histogram (Exec::hgram), zero between selects
   std::mutex mu;
   // stage timing (ggks_ctx_profile_arm)
   std::vector<hipEvent_t> prof;  // nslots * (GGKS_NSTAGES + 1)
@@ -155,6 +156,8 @@ static int ctx_init(ggks_ctx* c, int device, hipStream_t stream) {
   HIP_TRY(hipSetDevice(device));
   HIP_TRY(hipMalloc(&c->status, 256));
   HIP_TRY(hipMemset(c->status, 0, 256));
+  HIP_TRY(hipMalloc(&c->hgram, (size_t)4 << kMaxPrefixBits));
+  HIP_TRY(hipMemset(c->hgram, 0, (size_t)4 << kMaxPrefixBits));
   return GGKS_OK;
 }
 
@@ -232,6 +235,10 @@ struct Exec {
   // children are finished beside the engine pipeline
   hipStream_t side = nullptr;
   hipEvent_t fork = nullptr, join = nullptr;
+  // optional: a zeroed histogram of 2^kMaxPrefixBits words for a plan of one
+  // sampled segment (k_sample counts the sample into it, k_lut reads it and
+  // clears it again), owned by the context
+  uint32_t* hgram = nullptr;
 };
 
 static void stage(const Exec& x, int st) {
@@ -351,18 +358,20 @@ static int enqueue_level(const Exec& x, const Plan& p, const KT* d_src, KT* d_ou
     Query* cq2 = p.level2 ? reinterpret_cast<Query*>(A + p.off_cq2) : nullptr;
     KT* red2 = p.level2 ? reinterpret_cast<KT*>(A + p.off_red2) : nullptr;
 
+    // the context's sample histogram: one segment, and k_sample runs
+    uint32_t* hg = (nseg == 1 && p.nsblocks > 0) ? x.hgram : nullptr;
     stage(x, 1);
     if (p.nsblocks > 0) {
       const SampleBlock* sblocks = reinterpret_cast<const SampleBlock*>(A + p.off_sblocks);
       hipLaunchKernelGGL((k_sample<KT, C>), dim3(p.nsblocks), dim3(kSampleThreads), 0, s,
-                         d_src, segs, sblocks, samples, minmax, p.seed, states, lut);
+                         d_src, segs, sblocks, samples, minmax, p.seed, states, lut, hg);
     }
     stage(x, 2);
     uint32_t* bigw = reinterpret_cast<uint32_t*>(A + p.off_big);
     uint2* cinfo = reinterpret_cast<uint2*>(A + p.off_cinfo);
     hipLaunchKernelGGL((k_lut<KT>), dim3(nseg, p.lut_parts), dim3(kThreads), 0, s, segs, samples,
                        minmax, states,
-                       lut, l2count, bigw);
+                       lut, l2count, bigw, hg);
     stage(x, 3);
     if (beside) {  // nseg > 0
       HIP_TRY(hipEventRecord(x.fork, s));
@@ -535,7 +544,8 @@ static int select_one(ggks_ctx* c, int kind, int dtype, const void* d_vec, int64
                          d_q, c->status);
   }
   void* d_out = host_out ? (void*)(c->arena + off_dout) : out;
-  const Exec x = {c->arena, c->stream, c->status, c->prof_cur >= 0 ? c : nullptr};
+  Exec x = {c->arena, c->stream, c->status, c->prof_cur >= 0 ? c : nullptr};
+  x.hgram = c->hgram;
   rc = launch(x, dtype, p, d_vec, d_out, d_k);
   {
     ggks_debug_regions& g = c->dbg;
@@ -799,6 +809,7 @@ int ggks_ctx_destroy(ggks_ctx* c) {
     if (s.ev) (void)hipEventDestroy(s.ev);
   }
   if (c->status) (void)hipFree(c->status);
+  if (c->hgram) (void)hipFree(c->hgram);
   delete c;
   return GGKS_OK;
 }
