@@ -528,7 +528,7 @@ __global__ void k_topk_queries(uint32_t K, uint64_t n, Query* __restrict__ q) {
 // candidate with its count: every wavefront proposes its first sample, the
 // block counts each proposal's copies among its 1024 samples, and keeps the
 // most frequent (the smaller key on a tie).
-constexpr uint32_t kSampleThreads = 256;
+constexpr uint32_t kSampleThreads = kThreads;  // one sample a thread
 constexpr uint32_t kSamplePer = kThreads / kSampleThreads;
 constexpr uint32_t kSampleWaves = kSampleThreads / 64;
 template <typename KT, int C>
@@ -563,6 +563,15 @@ __global__ __launch_bounds__(kSampleThreads) void k_sample(const KT* __restrict_
     z ^= z >> 31;
     k[j] = src[sd.data_off + __umul64hi(z, sd.len)];
   }
+  // With a histogram (one sampled segment) the sample is also counted by
+  // the top pbits bits of the key, into hgram: where the sampled range spans
+  // the top bit (full-width ranges, such as floats of both signs) the LUT's
+  // prefixes are exactly these bins, and k_lut reads 2^pbits counts instead
+  // of histogramming the whole sample on one CU.
+  const bool hg = hgram != nullptr && sd.mode == MODE_SAMPLE;  // uniform
+  const uint32_t hsh = KeyBits<KT>::W - sd.pbits;
+  if (hg)  // zeroed while the gathers are in flight
+    for (uint32_t i = threadIdx.x; i < (1u << sd.pbits); i += kSampleThreads) hl[i] = 0;
   KT lo = KeyBits<KT>::MAX, hi = 0;
 #pragma unroll
   for (uint32_t j = 0; j < kSamplePer; ++j) {
@@ -571,19 +580,6 @@ __global__ __launch_bounds__(kSampleThreads) void k_sample(const KT* __restrict_
     samples[sd.sample_off + sb.blk * kThreads + j * kSampleThreads + threadIdx.x] = x;
     lo = x < lo ? x : lo;
     hi = x > hi ? x : hi;
-  }
-  // With a histogram (one sampled segment) the sample is also counted by
-  // the top pbits bits of the key, into hgram: where the sampled range spans
-  // the top bit (full-width ranges, such as floats of both signs) the LUT's
-  // prefixes are exactly these bins, and k_lut reads 2^pbits counts instead
-  // of histogramming the whole sample on one CU.
-  const bool hg = hgram != nullptr && sd.mode == MODE_SAMPLE;  // uniform
-  const uint32_t hsh = KeyBits<KT>::W - sd.pbits;
-  if (hg) {
-    for (uint32_t i = threadIdx.x; i < (1u << sd.pbits); i += kSampleThreads) hl[i] = 0;
-    __syncthreads();
-#pragma unroll
-    for (uint32_t j = 0; j < kSamplePer; ++j) atomicAdd(&hl[(uint32_t)(k[j] >> hsh)], 1u);
   }
   if (lane == 0) cand[w] = k[0];
   for (int d = 32; d >= 1; d >>= 1) {
@@ -595,37 +591,51 @@ __global__ __launch_bounds__(kSampleThreads) void k_sample(const KT* __restrict_
     red_lo[w] = lo;
     red_hi[w] = hi;
   }
-  __syncthreads();
+  __syncthreads();  // cand, red_lo / red_hi and the zeroed hl
+  if (hg) {
+#pragma unroll
+    for (uint32_t j = 0; j < kSamplePer; ++j) atomicAdd(&hl[(uint32_t)(k[j] >> hsh)], 1u);
+  }
+  // copies of every candidate in this wavefront's samples (ballot counts)
 #pragma unroll
   for (uint32_t c = 0; c < kSampleWaves; ++c) {
     const KT cv = cand[c];
     uint32_t m = 0;
 #pragma unroll
-    for (uint32_t j = 0; j < kSamplePer; ++j) m += k[j] == cv ? 1u : 0u;
-    m = wave_last(wave_incl_scan(m));
+    for (uint32_t j = 0; j < kSamplePer; ++j) m += (uint32_t)__popcll(__ballot(k[j] == cv));
     if (lane == 0) ccnt[w][c] = m;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    for (uint32_t v = 1; v < kSampleWaves; ++v) {
-      lo = red_lo[v] < lo ? red_lo[v] : lo;
-      hi = red_hi[v] > hi ? red_hi[v] : hi;
+  if (w == 0) {
+    // lane c < kSampleWaves: candidate c's copies in the block, and the
+    // block's min and max; the most frequent candidate (the smaller key on
+    // a tie) by a wavefront reduction
+    uint32_t t = 0;
+    KT cv = KeyBits<KT>::MAX, l = KeyBits<KT>::MAX, h = 0;
+    if (lane < kSampleWaves) {
+      for (uint32_t v = 0; v < kSampleWaves; ++v) t += ccnt[v][lane];
+      cv = cand[lane];
+      l = red_lo[lane];
+      h = red_hi[lane];
     }
-    KT best = cand[0];
-    uint32_t bn = 0;
-    for (uint32_t c = 0; c < kSampleWaves; ++c) {
-      uint32_t t = 0;
-      for (uint32_t v = 0; v < kSampleWaves; ++v) t += ccnt[v][c];
-      if (t > bn || (t == bn && cand[c] < best)) {
-        bn = t;
-        best = cand[c];
+    for (int d = 32; d >= 1; d >>= 1) {
+      const KT la = __shfl_xor(l, d, 64), ha = __shfl_xor(h, d, 64);
+      l = la < l ? la : l;
+      h = ha > h ? ha : h;
+      const uint32_t ot = (uint32_t)__shfl_xor((int)t, d, 64);
+      const KT oc = __shfl_xor(cv, d, 64);
+      if (ot > t || (ot == t && oc < cv)) {
+        t = ot;
+        cv = oc;
       }
     }
-    KT* mm = minmax + 4 * (sd.sample_off / kThreads + sb.blk);
-    mm[0] = lo;
-    mm[1] = hi;
-    mm[2] = best;
-    mm[3] = (KT)bn;
+    if (lane == 0) {
+      KT* mm = minmax + 4 * (sd.sample_off / kThreads + sb.blk);
+      mm[0] = l;
+      mm[1] = h;
+      mm[2] = cv;
+      mm[3] = (KT)t;
+    }
   }
   if (hg) {  // hl is complete: the barrier above follows every count
     for (uint32_t i = threadIdx.x; i < (1u << sd.pbits); i += kSampleThreads)
