@@ -733,6 +733,10 @@ __global__ __launch_bounds__(kThreads) void k_lut(const SegDesc* __restrict__ se
   constexpr uint32_t W = KeyBits<KT>::W;
   const SegDesc sd = segs[blockIdx.x];
   const uint32_t tid = threadIdx.x;
+  // k_sample's prefix counts, loaded now (16 bytes a thread) so that they
+  // arrive while the sample's range is reduced; used for full-width ranges
+  const bool hload = hgram && sd.mode == MODE_SAMPLE && tid < (1u << sd.pbits) / 4;
+  const uint4 h4 = hload ? reinterpret_cast<const uint4*>(hgram)[tid] : make_uint4(0, 0, 0, 0);
   // part blockIdx.y of the segment's LUT build: its parts, at most the
   // grid's y extent (1 keeps every segment's build in one work-group)
   const uint32_t nparts = min(lut_parts(sd.mode, sd.nsample, sizeof(KT)), gridDim.y), part = blockIdx.y;
@@ -857,8 +861,10 @@ __global__ __launch_bounds__(kThreads) void k_lut(const SegDesc* __restrict__ se
       }
     };
     if (useh) {
+      if (hload) reinterpret_cast<uint4*>(cnt4)[tid] = h4;  // the counts by key >> s
+      __syncthreads();
       const uint32_t h0 = (uint32_t)(kb >> s);
-      for (uint32_t p = tid; p < P; p += kThreads) cnt[p] = hgram[h0 + p];
+      for (uint32_t p = tid; p < P; p += kThreads) cnt[p] = cnt4[h0 + p];
       __syncthreads();
     } else if (nparts == 1) {
       // the whole sample in one round of loads (64 per thread for 4-byte keys)
@@ -899,10 +905,10 @@ __global__ __launch_bounds__(kThreads) void k_lut(const SegDesc* __restrict__ se
       if (cnt[p]) nonempty++;
       else if (p == 0 || cnt[p - 1]) runs++;
     }
-    uint32_t t1, t2, t3;
-    block_excl_scan(nonempty, wsum, t1);
-    block_excl_scan(runs, wsum, t2);
+    uint32_t t12, t3;  // nonempty prefixes and empty runs (< 2^16 each) in one scan
+    block_excl_scan(nonempty | (runs << 16), wsum, t12);
     block_excl_scan(csum, wsum, t3);
+    const uint32_t t1 = t12 & 0xFFFFu, t2 = t12 >> 16;
     const uint32_t A = Bt > t1 + t2 ? Bt - t1 - t2 : 0u;
     // the counts' own total (the sample size; the sub-bucket shares below
     // stay within the budget whatever the counts sum to)
