@@ -985,11 +985,6 @@ __global__ __launch_bounds__(kThreads) void k_count(const KT* __restrict__ src,
   const uint32_t bmax = sd.bmax;
   uint32_t* hist = lds;
   uint32_t* lut = lds + bmax;
-  for (uint32_t i = threadIdx.x; i < st.B; i += kThreads) hist[i] = 0;
-  for (uint32_t i = threadIdx.x; i < st.P; i += kThreads) lut[i] = lut_pool[sd.lut_off + i];
-  __syncthreads();
-  BucketFn<KT> bk;
-  bk.init(st);
   constexpr uint32_t VEC = 16 / sizeof(KT);
   constexpr int R = 4, N = R * VEC;
   const KT* p = src + sd.data_off + ch.start;
@@ -1000,11 +995,23 @@ __global__ __launch_bounds__(kThreads) void k_count(const KT* __restrict__ src,
   const uint64_t nvec = (len - head) / VEC;
   const uint64_t done = head + nvec * VEC;
   const uint32_t tid = threadIdx.x;
-  if (tid < head) atomicAdd(&hist[bk(to_key<C>(p[tid]), lut)], 1u);
-  if (done + tid < len) atomicAdd(&hist[bk(to_key<C>(p[done + tid]), lut)], 1u);
   const uint4* __restrict__ pv = reinterpret_cast<const uint4*>(p + head);
   const uint64_t per_round = (uint64_t)R * kThreads;
   uint64_t r0 = 0;
+  // the first round's loads go out before the histogram is cleared and the
+  // LUT loaded, so they arrive during that prologue
+  uint4 v[R];
+  if (per_round <= nvec) {  // uniform
+#pragma unroll
+    for (int t = 0; t < R; ++t) v[t] = stream_load(&pv[(uint64_t)t * kThreads + tid]);
+  }
+  for (uint32_t i = threadIdx.x; i < st.B; i += kThreads) hist[i] = 0;
+  for (uint32_t i = threadIdx.x; i < st.P; i += kThreads) lut[i] = lut_pool[sd.lut_off + i];
+  __syncthreads();
+  BucketFn<KT> bk;
+  bk.init(st);
+  if (tid < head) atomicAdd(&hist[bk(to_key<C>(p[tid]), lut)], 1u);
+  if (done + tid < len) atomicAdd(&hist[bk(to_key<C>(p[done + tid]), lut)], 1u);
   // full rounds, software pipelined: the next round's loads are in flight
   // while a round is bucketed and counted
   // With a heavy value its bucket takes up to most keys: those are counted
@@ -1013,10 +1020,7 @@ __global__ __launch_bounds__(kThreads) void k_count(const KT* __restrict__ src,
   const uint32_t hbk = st.hb;
   uint32_t hvc = 0;  // wave-uniform
   auto full_rounds = [&](auto left, auto hvt) {
-    if (r0 + per_round > nvec) return;  // uniform
-    uint4 v[R];
-#pragma unroll
-    for (int t = 0; t < R; ++t) v[t] = stream_load(&pv[r0 + (uint64_t)t * kThreads + tid]);
+    if (r0 + per_round > nvec) return;  // uniform (round 0's loads are in v)
     for (; r0 + per_round <= nvec; r0 += per_round) {
       KT k[N];
       unpack<KT, R>(v, k);
@@ -1539,6 +1543,41 @@ __device__ __forceinline__ void compact_chunk(const KT* __restrict__ src,
   uint32_t* cur = lds;
   KT* kb = reinterpret_cast<KT*>(cur + kCompactCursors);
   const uint32_t tid = threadIdx.x, lane = lane_id();
+  const KT* p = src + sd.data_off + ch.start;
+  const uint64_t len = ch.len;
+  uint64_t mis = ((uintptr_t)p & 15u) / sizeof(KT);
+  uint64_t head = mis ? (VEC - mis) : 0;
+  if (head > len) head = len;
+  const uint64_t nvec = (len - head) / VEC;
+  const uint64_t done = head + nvec * VEC;
+  const uint4* __restrict__ pv = reinterpret_cast<const uint4*>(p + head);
+  const uint64_t per_round = (uint64_t)kRound * kThreads;
+  const uint64_t rounds = (nvec + per_round - 1) / per_round;  // uniform
+  const uint64_t full_rounds = nvec / per_round;  // rounds with every load in range
+  // software pipeline: the loads of round r + 1 are in flight while round r
+  // is filtered and appended
+  uint4 v[kRound];
+  uint32_t okmask = 0;
+  auto load_round = [&](uint64_t r) {
+    const uint4* q = pv + r * per_round + tid;
+    if (r < full_rounds) {  // uniform
+      okmask = (N >= 32) ? 0xFFFFFFFFu : ((1u << N) - 1u);
+#pragma unroll
+      for (int t = 0; t < kRound; ++t) v[t] = stream_load(q + (uint64_t)t * kThreads);
+      return;
+    }
+    okmask = 0;
+#pragma unroll
+    for (int t = 0; t < kRound; ++t) {
+      const uint64_t idx = r * per_round + (uint64_t)t * kThreads + tid;
+      const bool ok = idx < nvec;
+      okmask |= ok ? (((1u << VEC) - 1u) << (t * VEC)) : 0u;
+      v[t] = ok ? stream_load(q + (uint64_t)t * kThreads) : make_uint4(0, 0, 0, 0);
+    }
+  };
+  // round 0's loads go out before the tables are loaded and the cursors
+  // set, so they arrive during that prologue
+  if (rounds) load_round(0);
   strided_for(lut_pool + sd.lut_off, st.P, [&](uint32_t i, uint32_t x) { lut[i] = x; });
   for (uint32_t i = tid; i < nwords; i += kThreads) {
     const uint32_t w = bits_pool[sd.bits_off + i];
@@ -1605,42 +1644,9 @@ __device__ __forceinline__ void compact_chunk(const KT* __restrict__ src,
     if (tid == 0) nbuf = 0;
     __syncthreads();
   };
-  const KT* p = src + sd.data_off + ch.start;
-  const uint64_t len = ch.len;
-  uint64_t mis = ((uintptr_t)p & 15u) / sizeof(KT);
-  uint64_t head = mis ? (VEC - mis) : 0;
-  if (head > len) head = len;
-  const uint64_t nvec = (len - head) / VEC;
-  const uint64_t done = head + nvec * VEC;
   // head and tail: fewer than VEC keys each (a thread may own one of each)
   if (tid < head) place(p[tid]);
   if (done + tid < len) place(p[done + tid]);
-  const uint4* __restrict__ pv = reinterpret_cast<const uint4*>(p + head);
-  const uint64_t per_round = (uint64_t)kRound * kThreads;
-  const uint64_t rounds = (nvec + per_round - 1) / per_round;  // uniform
-  const uint64_t full_rounds = nvec / per_round;  // rounds with every load in range
-  // software pipeline: the loads of round r + 1 are in flight while round r
-  // is filtered and appended
-  uint4 v[kRound];
-  uint32_t okmask = 0;
-  auto load_round = [&](uint64_t r) {
-    const uint4* q = pv + r * per_round + tid;
-    if (r < full_rounds) {  // uniform
-      okmask = (N >= 32) ? 0xFFFFFFFFu : ((1u << N) - 1u);
-#pragma unroll
-      for (int t = 0; t < kRound; ++t) v[t] = stream_load(q + (uint64_t)t * kThreads);
-      return;
-    }
-    okmask = 0;
-#pragma unroll
-    for (int t = 0; t < kRound; ++t) {
-      const uint64_t idx = r * per_round + (uint64_t)t * kThreads + tid;
-      const bool ok = idx < nvec;
-      okmask |= ok ? (((1u << VEC) - 1u) << (t * VEC)) : 0u;
-      v[t] = ok ? stream_load(q + (uint64_t)t * kThreads) : make_uint4(0, 0, 0, 0);
-    }
-  };
-  if (rounds) load_round(0);
   auto main_loop = [&](auto left, auto hvt) {
     for (uint64_t r = 0; r < rounds; ++r) {
       KT k[N];
