@@ -78,8 +78,10 @@ struct ggks_ctx {
   int prof_slots = 0;
   uint64_t prof_calls = 0;
   int prof_cur = -1;  // slot of the call in flight, -1 when not armed
-  uint32_t prof_mask = ~0u;   // events to record (ggks_ctx_profile_marks)
-  uint32_t prof_armed = ~0u;  // the mask latched by the last arm: recorded and read
+  // events to record (ggks_ctx_profile_marks; default: every stage's, as
+  // markers), and the mask latched by the last arm: recorded and read
+  uint32_t prof_mask = (1u << (GGKS_NSTAGES + 1)) - 1u;
+  uint32_t prof_armed = (1u << (GGKS_NSTAGES + 1)) - 1u;
   // plan-table cache: the tables in the arena match this signature
   uint64_t table_sig = 0;
   ggks_debug_regions dbg = {};  // the last select's tables (ggks_ctx_debug_regions)
@@ -871,6 +873,10 @@ int ggks_ctx_profile_read(ggks_ctx* c, float* ms, int nslots) {
     float* row = ms + (size_t)sl * per;
     for (int st = 0; st <= GGKS_NSTAGES; ++st) row[st] = 0.f;
     for (int st = 0; st < GGKS_NSTAGES; ++st) {
+      if (kev) {  // a kernel-timed stage: its launch's own two events
+        if (kpair(st)) HIP_TRY(hipEventElapsedTime(&row[st], e[st], e[st + 1]));
+        continue;
+      }
       if (!recorded(st)) continue;
       int nx = st + 1;
       while (nx <= GGKS_NSTAGES && !recorded(nx)) ++nx;

@@ -180,6 +180,16 @@ def test_stage_events_and_marks(pkg, O):
     ctx = pkg.Context(0)
     per = L.NSTAGES + 1
     ci = L.STAGES.index("compact")
+    # a new context records every stage's events as markers: the stages sum
+    # to the whole call
+    L.check(lib.ggks_ctx_profile_arm(ctx.handle, 2))
+    for _ in range(2):
+        pkg.multiselect_async(d, dk, dout, ctx)
+    ms = np.zeros(2 * per, np.float32)
+    L.check(lib.ggks_ctx_profile_read(ctx.handle, ms.ctypes.data, 2))
+    for row in ms.reshape(2, per):
+        assert row[ci] > 0 and row[L.STAGES.index("count")] > 0
+        assert abs(row[:L.NSTAGES].sum() - row[L.NSTAGES]) <= 1e-3 * row[L.NSTAGES] + 1e-3
     for mask in ((1 << per) - 1, (1 << ci) | (1 << (ci + 1))):
         L.check(lib.ggks_ctx_profile_marks(ctx.handle, mask))
         L.check(lib.ggks_ctx_profile_arm(ctx.handle, 2))
