@@ -971,6 +971,7 @@ __global__ __launch_bounds__(kThreads) void k_lut(const SegDesc* __restrict__ se
 }
 
 // Pass 1: per-chunk LDS histogram of bucket ids.
+constexpr uint32_t kLaneSlots = 64;  // k_count's per-lane heavy-bin slots
 template <typename KT, int C>
 __global__ __launch_bounds__(kThreads) void k_count(const KT* __restrict__ src,
                                                     const SegDesc* __restrict__ segs,
@@ -983,8 +984,11 @@ __global__ __launch_bounds__(kThreads) void k_count(const KT* __restrict__ src,
   const SegDesc& sd = segs[ch.seg];
   const SegState st = states[ch.seg];
   const uint32_t bmax = sd.bmax;
+  // LDS: the histogram (bmax bins), one slot per lane for the heavy value's
+  // bin (kLaneSlots), the LUT
   uint32_t* hist = lds;
-  uint32_t* lut = lds + bmax;
+  const uint32_t hslot = bmax + lane_id();  // this lane's heavy-bin slot
+  uint32_t* lut = lds + bmax + kLaneSlots;
   constexpr uint32_t VEC = 16 / sizeof(KT);
   constexpr int R = 4, N = R * VEC;
   const KT* p = src + sd.data_off + ch.start;
@@ -1006,6 +1010,7 @@ __global__ __launch_bounds__(kThreads) void k_count(const KT* __restrict__ src,
     for (int t = 0; t < R; ++t) v[t] = stream_load(&pv[(uint64_t)t * kThreads + tid]);
   }
   for (uint32_t i = threadIdx.x; i < st.B; i += kThreads) hist[i] = 0;
+  if (threadIdx.x < kLaneSlots) hist[bmax + threadIdx.x] = 0;
   for (uint32_t i = threadIdx.x; i < st.P; i += kThreads) lut[i] = lut_pool[sd.lut_off + i];
   __syncthreads();
   BucketFn<KT> bk;
@@ -1014,11 +1019,11 @@ __global__ __launch_bounds__(kThreads) void k_count(const KT* __restrict__ src,
   if (done + tid < len) atomicAdd(&hist[bk(to_key<C>(p[done + tid]), lut)], 1u);
   // full rounds, software pipelined: the next round's loads are in flight
   // while a round is bucketed and counted
-  // With a heavy value its bucket takes up to most keys: those are counted
-  // per wavefront (ballot and popcount, one LDS atomic per wavefront at the
-  // end) instead of one same-address LDS atomic per key.
+  // With a heavy value its bucket takes up to most keys: each lane counts
+  // those in a slot of its own (a select of the address, no lanes of one
+  // atomic on one address, no change of the execution mask), and the slots
+  // are added into the bucket at the end.
   const uint32_t hbk = st.hb;
-  uint32_t hvc = 0;  // wave-uniform
   auto full_rounds = [&](auto left, auto hvt) {
     if (r0 + per_round > nvec) return;  // uniform (round 0's loads are in v)
     for (; r0 + per_round <= nvec; r0 += per_round) {
@@ -1033,11 +1038,7 @@ __global__ __launch_bounds__(kThreads) void k_count(const KT* __restrict__ src,
       bucket_batch1<KT, C, N, decltype(left)::value, decltype(hvt)::value>(bk, lut, k, b);
       if (decltype(hvt)::value) {
 #pragma unroll
-        for (int i = 0; i < N; ++i) {
-          const bool h = b[i] == hbk;
-          hvc += (uint32_t)__popcll(__ballot(h));
-          if (!h) atomicAdd(&hist[b[i]], 1u);
-        }
+        for (int i = 0; i < N; ++i) atomicAdd(&hist[b[i] == hbk ? hslot : b[i]], 1u);
       } else {
 #pragma unroll
         for (int i = 0; i < N; ++i) atomicAdd(&hist[b[i]], 1u);
@@ -1061,7 +1062,14 @@ __global__ __launch_bounds__(kThreads) void k_count(const KT* __restrict__ src,
 #pragma unroll
     for (uint32_t j = 0; j < VEC; ++j) atomicAdd(&hist[b[j]], 1u);
   }
-  if (st.hvon && lane_id() == 0 && hvc) atomicAdd(&hist[hbk], hvc);
+  if (st.hvon) {  // uniform
+    __syncthreads();
+    if (threadIdx.x < kLaneSlots) {
+      const uint32_t v = hist[bmax + threadIdx.x];
+      const uint32_t t = wave_last(wave_incl_scan(v));
+      if (threadIdx.x == 0) hist[hbk] += t;
+    }
+  }
   __syncthreads();
   uint32_t* row = rows + sd.row_base + (uint64_t)ch.idx * bmax;
   for (uint32_t i = threadIdx.x; i < st.B; i += kThreads) row[i] = hist[i];

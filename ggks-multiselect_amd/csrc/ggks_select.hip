@@ -254,7 +254,7 @@ static void stage(const Exec& x, int st) {
 template <typename KT, int C>
 static void level_attrs(const Plan&) {
   static const bool once = [] {
-    set_lds_limit(k_count<KT, C>, ((size_t)kMaxBuckets + (1u << kMaxPrefixBits)) * 4);
+    set_lds_limit(k_count<KT, C>, ((size_t)kMaxBuckets + kLaneSlots + (1u << kMaxPrefixBits)) * 4);
     set_lds_limit(k_findk<KT, C>, ((size_t)kMaxBuckets + (1u << kMaxPrefixBits) +
                                    3 * ((kMaxBuckets + 31) / 32) + 32) * 4);
     set_lds_limit(k_findk_small<KT, C>, ((size_t)kFindkSmallBuckets + (1u << kMaxPrefixBits) +
@@ -382,7 +382,7 @@ static int enqueue_level(const Exec& x, const Plan& p, const KT* d_src, KT* d_ou
       HIP_TRY(hipEventRecord(x.join, x.side));
     }
     launch_stage(x.prof, 3, k_count<KT, C>, dim3(nchunks), dim3(kThreads),
-                 (uint32_t)(((size_t)p.bmax_max + (1u << p.pbits_max)) * 4), s, d_src, segs, states,
+                 (uint32_t)(((size_t)p.bmax_max + kLaneSlots + (1u << p.pbits_max)) * 4), s, d_src, segs, states,
                  chunks, lut, rows);
     stage(x, 4);
     hipLaunchKernelGGL(k_reduce, dim3(p.nitems), dim3(kThreads), 0, s, segs,
