@@ -1445,6 +1445,7 @@ __device__ __forceinline__ void findk_segment(const SegDesc* __restrict__ segs,
   }
   if (tid == 0) {
     states[blockIdx.x].nact = U;
+    states[blockIdx.x].dense = (U <= kDenseChildren && ((uint64_t)mtot << kDenseLog2) >= sd.len) ? 1u : 0u;
     if (bigw && lbig) atomicAdd(bigw, lbig);
   }
 }
@@ -1486,19 +1487,29 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) v
 // holds more than a few thousand keys (or could not take another round) it
 // is flushed: every buffered key is placed (child, LDS cursor, store) in one
 // short burst between rounds.
-constexpr uint32_t kCompactCursors = 1024;  // LDS cursors; children beyond use global atomics
-
 // Table sizes of k_compact.  The full instance serves any plan.  The small
 // one serves plans of at most 4096 buckets and 2^10 LUT prefixes per segment
 // in under 62 KB of LDS, so two work-groups share a CU and one chunk's table
 // loads and last flush hide behind another chunk's streaming (batches of
 // many short segments, whose chunks are 2^14..2^17 keys).
+//
+// The dynamic LDS holds the children's cursors and the key buffer.  The
+// first `cursors` children of a segment keep their cursor in LDS; children
+// beyond place each key with a global atomic on the chunk's row.  With
+// 4-byte keys the full instance's buffer still takes a round above the
+// flush threshold with 4096 cursors, so it has room for them (selects of a
+// few thousand ranks keep every cursor in LDS); 8-byte keys need the room
+// for the buffer.
 template <bool SMALL>
 struct CompactCfg {
   static constexpr uint32_t lut_words = SMALL ? (1u << 10) : (1u << kMaxPrefixBits);
   static constexpr uint32_t buckets = SMALL ? 4096u : kMaxBuckets;
   static constexpr int round = SMALL ? 2 : 4;  // 16-byte loads per lane per round
-  static constexpr uint32_t buf_bytes = SMALL ? 48u * 1024 : 96u * 1024;  // buffered active keys
+  static constexpr uint32_t dyn_bytes = SMALL ? 52u * 1024 : 100u * 1024;  // cursors + key buffer
+  template <typename KT>
+  static constexpr uint32_t cursors = (!SMALL && sizeof(KT) == 4) ? 4096u : 1024u;
+  template <typename KT>
+  static constexpr uint32_t buf_bytes = dyn_bytes - cursors<KT> * 4;  // buffered active keys
 };
 inline bool compact_small(uint32_t pbits_max, uint32_t bmax_max) {
   return (1u << pbits_max) <= CompactCfg<true>::lut_words && bmax_max <= CompactCfg<true>::buckets;
@@ -1507,7 +1518,7 @@ inline bool compact_small(uint32_t pbits_max, uint32_t bmax_max) {
 template <bool SMALL>
 __host__ __device__ constexpr size_t compact_lds_bytes() {
   // dynamic part: cursors and the key buffer (the LUT and bitmaps are static)
-  return (size_t)kCompactCursors * 4 + CompactCfg<SMALL>::buf_bytes;
+  return CompactCfg<SMALL>::dyn_bytes;
 }
 
 template <typename KT, int C, bool SMALL>
@@ -1530,6 +1541,7 @@ __device__ __forceinline__ void compact_chunk(const KT* __restrict__ src,
   __shared__ uint32_t wbase[Cfg::buckets / 32];  // first child of each bitmap word
   __shared__ uint32_t abyte[Cfg::buckets / 4];   // active buckets, a byte (0 or 1) each
   __shared__ uint32_t nbuf;                      // keys in the buffer
+  __shared__ uint32_t dsum[kThreads / 64];       // dense rounds: wavefront totals
   const Chunk ch = chunks[blockIdx.x];
   const SegDesc& sd = segs[ch.seg];
   const SegState st = states[ch.seg];
@@ -1537,7 +1549,8 @@ __device__ __forceinline__ void compact_chunk(const KT* __restrict__ src,
   if (U == 0) return;
   constexpr uint32_t VEC = 16 / sizeof(KT);
   constexpr int N = kRound * VEC;  // keys per lane per round
-  constexpr uint32_t CAP = Cfg::buf_bytes / sizeof(KT);
+  constexpr uint32_t CURS = Cfg::template cursors<KT>;
+  constexpr uint32_t CAP = Cfg::template buf_bytes<KT> / sizeof(KT);
   constexpr uint32_t ROUND_MAX = kThreads * N;  // most keys one round can append
   // a round starts with at most CAP - ROUND_MAX keys buffered, so it fits
   static_assert(CAP > ROUND_MAX, "buffer must hold more than a round");
@@ -1549,7 +1562,7 @@ __device__ __forceinline__ void compact_chunk(const KT* __restrict__ src,
   constexpr uint32_t FLUSH_AT = CAP - ROUND_MAX < 4096u ? CAP - ROUND_MAX : 4096u;
   const uint32_t nwords = (st.B + 31) / 32;
   uint32_t* cur = lds;
-  KT* kb = reinterpret_cast<KT*>(cur + kCompactCursors);
+  KT* kb = reinterpret_cast<KT*>(cur + CURS);
   const uint32_t tid = threadIdx.x, lane = lane_id();
   const KT* p = src + sd.data_off + ch.start;
   const uint64_t len = ch.len;
@@ -1604,7 +1617,7 @@ __device__ __forceinline__ void compact_chunk(const KT* __restrict__ src,
   // however far into the buffer the segment sits (batches and served groups
   // of more than 2^32 keys).
   KT* const rseg = red + sd.red_base;
-  for (uint32_t u = tid; u < U && u < kCompactCursors; u += kThreads)
+  for (uint32_t u = tid; u < U && u < CURS; u += kThreads)
     cur[u] = kinfo[u].x + row[kinfo[u].y];
   if (tid == 0) nbuf = 0;
   __syncthreads();
@@ -1617,7 +1630,7 @@ __device__ __forceinline__ void compact_chunk(const KT* __restrict__ src,
     const uint32_t sh = b & 31u;
     if ((bits >> sh) & 1u) {
       const uint32_t u = wbase[b >> 5] + __popc(bits & ((1u << sh) - 1u));
-      const uint32_t pos = u < kCompactCursors
+      const uint32_t pos = u < CURS
                                ? atomicAdd(&cur[u], 1u)
                                : kinfo[u].x + atomicAdd(&row[kinfo[u].y], 1u);
       rseg[pos] = raw;
@@ -1641,7 +1654,7 @@ __device__ __forceinline__ void compact_chunk(const KT* __restrict__ src,
       for (int f = 0; f < F; ++f) u[f] = wbase[bb[f] >> 5] + __popc(w[f] & ((1u << (bb[f] & 31u)) - 1u));
 #pragma unroll
       for (int f = 0; f < F; ++f) {
-        const uint32_t pos = u[f] < kCompactCursors
+        const uint32_t pos = u[f] < CURS
                                  ? atomicAdd(&cur[u[f]], 1u)
                                  : kinfo[u[f]].x + atomicAdd(&row[kinfo[u[f]].y], 1u);
         rseg[pos] = kk[f];
@@ -1688,6 +1701,97 @@ __device__ __forceinline__ void compact_chunk(const KT* __restrict__ src,
       if (nb > FLUSH_AT) flush(nb);  // uniform
     }
   };
+  // Dense segments (most keys active): a key-per-line scatter would cost one
+  // L2 write request per key.  Instead each round's active keys are counted
+  // per child (the LDS atomic's return value is the key's rank in the
+  // child), the children's counts are scanned, the keys are written to LDS
+  // in child order, and the sorted round is stored with consecutive
+  // threads on consecutive slots of a child, so a wavefront's stores cover
+  // a few lines.  Children beyond the LDS cursors place their keys directly.
+  auto dense_loop = [&](auto left, auto hvt) {
+    if constexpr (!SMALL) {
+      constexpr uint32_t NONE32 = 0xFFFFFFFFu;
+      constexpr uint32_t DC = kDenseChildren;
+      uint32_t* ts = reinterpret_cast<uint32_t*>(kb);    // child's first slot in the sorted round
+      uint32_t* dl = ts + DC;                             // child's slice position - first slot
+      uint32_t* cnt = dl + DC;                            // child's keys in the round
+      KT* srt = reinterpret_cast<KT*>(cnt + DC);          // the round's keys, by child
+      uint16_t* su = reinterpret_cast<uint16_t*>(abyte);  // child of each sorted key
+      static_assert(3 * DC * 4 + ROUND_MAX * sizeof(KT) <= Cfg::template buf_bytes<KT>,
+                    "dense round must fit the key buffer");
+      static_assert(ROUND_MAX * 2 <= sizeof(abyte), "dense round's child ids must fit the byte table");
+      static_assert(ROUND_MAX <= 65536 && DC <= CURS && DC <= kThreads, "16-bit ranks, children");
+      const uint32_t UC = U < DC ? U : DC;
+      const uint32_t wid = tid >> 6;
+      if (tid < UC) cnt[tid] = 0;
+      __syncthreads();
+      for (uint64_t r = 0; r < rounds; ++r) {
+        KT k[N];
+        unpack<KT, kRound>(v, k);
+        if (r + 1 < rounds) load_round(r + 1);
+        uint32_t b[N];
+        bucket_batch1<KT, C, N, decltype(left)::value, decltype(hvt)::value>(bk, lut, k, b);
+        const bool part = r >= full_rounds;  // uniform
+        // b[i] becomes (child << 16) | rank in the child, or NONE32.  (The
+        // counters were cleared before the previous round's last barrier,
+        // and every thread has finished that round's stores before any
+        // passes the next barrier, so three barriers a round suffice.)
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+          const uint32_t w = abits[b[i] >> 5], sh = b[i] & 31u;
+          const bool a = ((w >> sh) & 1u) && (!part || ((okmask >> i) & 1u));
+          const uint32_t u = wbase[b[i] >> 5] + __popc(w & ((1u << sh) - 1u));
+          uint32_t x = NONE32;
+          if (a) {
+            if (u < DC) {
+              x = (u << 16) | atomicAdd(&cnt[u], 1u);
+            } else {
+              rseg[kinfo[u].x + atomicAdd(&row[kinfo[u].y], 1u)] = k[i];
+            }
+          }
+          b[i] = x;
+        }
+        __syncthreads();  // counts final
+        const uint32_t c = tid < UC ? cnt[tid] : 0u;
+        const uint32_t inc = wave_incl_scan(c);
+        if (lane == 63) dsum[wid] = inc;
+        __syncthreads();  // wavefront totals written
+        const uint32_t ws = lane < kThreads / 64 ? dsum[lane] : 0u;
+        const uint32_t wi = wave_incl_scan(ws);
+        const uint32_t wpre = (uint32_t)__builtin_amdgcn_readlane((int)(wi - ws), (int)wid);
+        const uint32_t tot = (uint32_t)__builtin_amdgcn_readlane((int)wi, kThreads / 64 - 1);
+        if (tid < UC) {
+          const uint32_t s = wpre + inc - c;
+          ts[tid] = s;
+          dl[tid] = cur[tid] - s;
+          cur[tid] += c;
+          cnt[tid] = 0;
+        }
+        __syncthreads();  // first slots written
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+          if (b[i] != NONE32) {
+            const uint32_t u = b[i] >> 16;
+            const uint32_t j = ts[u] + (b[i] & 0xFFFFu);
+            srt[j] = k[i];
+            su[j] = (uint16_t)u;
+          }
+        }
+        __syncthreads();  // the round is sorted
+        for (uint32_t j = tid; j < tot; j += kThreads) rseg[dl[su[j]] + j] = srt[j];
+      }
+    }
+  };
+  if (!SMALL && st.dense) {  // uniform
+    if (!st.hvon) {
+      if (st.s <= 24) dense_loop(std::true_type{}, std::false_type{});
+      else dense_loop(std::false_type{}, std::false_type{});
+    } else {
+      if (st.s <= 24) dense_loop(std::true_type{}, std::true_type{});
+      else dense_loop(std::false_type{}, std::true_type{});
+    }
+    return;
+  }
   // uniform: the sub-field shift direction and the heavy-value term
   if (!st.hvon) {
     if (st.s <= 24) main_loop(std::true_type{}, std::false_type{});

@@ -9,7 +9,11 @@ fixes, not against a second copy of the kernels' arithmetic:
 - the bucket function is monotone in the key over every input key and the
   edges of its clamp range and heavy value;
 - a heavy-value bucket holds the heavy value alone, all of its copies;
-- chunk rows are exclusive prefix sums over chunks of per-chunk histograms;
+- chunk rows are exclusive prefix sums over chunks of per-chunk histograms
+  (the compaction then advances the rows of children past its LDS
+  cursors by the chunk's keys of that child);
+- the segment is flagged dense (compacted by child-sorted rounds) exactly
+  when it has at most 1024 children holding at least a quarter of its keys;
 - the children are the buckets of the active bitmap in order, packed
   contiguously, each with its exact count, a key range holding its keys
   (where one is planned), and a reduced slice equal to the multiset of its
@@ -33,9 +37,10 @@ from invariants import CHILD, CHUNK, QUERY, SEGDESC, SEGSTATE, bucket_of, device
 
 pytestmark = pytest.mark.gpu
 
-CASES = [  # (label, dtype, family, lo, hi, kind, n)
+CASES = [  # (label, dtype, family, lo, hi, kind, n[, number of k])
     ("f32-normal-sample", np.float32, "normal", 0, 0, "bucket", 1 << 22),
-    ("f32-normal-radix", np.float32, "normal", 0, 0, "radix", 1 << 22),
+    # most keys active: the compaction sorts each round by child in LDS
+    ("f32-normal-radix-dense", np.float32, "normal", 0, 0, "radix", 1 << 22),
     ("f32-plateau-heavy", np.float32, "plateau", 0, 0, "bucket", 1 << 22),
     ("f32-cauchy-sample", np.float32, "cauchy", 0, 0, "bucket", 1 << 22),
     ("u32-dup1024-sample", np.uint32, "uniform", 0, 1023, "bucket", 1 << 22),
@@ -46,17 +51,22 @@ CASES = [  # (label, dtype, family, lo, hi, kind, n)
     # over 4 and 3 work-groups (the last one taking a remainder)
     ("f64-normal-split-lut", np.float64, "normal", 0, 0, "bucket", 1 << 24),
     ("f64-uniform-split-lut-ragged", np.float64, "uniform", 0, 0, "bucket", 50176 * 256),
+    # more children than the dense sort's tables: 4-byte keys keep 4096
+    # children's cursors in LDS, the rest place through the chunk rows
+    ("u32-full-radix-4096k", np.uint32, "uniform", 0, 0xFFFFFFFF, "radix", 1 << 22, 4096),
+    ("u32-full-radix-16384k", np.uint32, "uniform", 0, 0xFFFFFFFF, "radix", 1 << 22, 16384),
 ]
 
 
 @pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
 def test_select_invariants(pkg, O, case):
-    label, dt, fam, lo, hi, kind, n = case
+    label, dt, fam, lo, hi, kind, n = case[:7]
+    nk = case[7] if len(case) > 7 else 512
     print(f"case invariants {label} n={n}", flush=True)
     lib = pkg._lib.lib()
     d = pkg.generate(dt, fam, 77, n, lo=lo, hi=hi)
     x = d.to_host()
-    k = pkg.workloads.k_uniform(78, n, 512)
+    k = pkg.workloads.k_uniform(78, n, nk)
     ctx = pkg.Context()
     got = pkg.multiselect(d, k, ctx=ctx, kind=kind, level2=-1)
     assert np.array_equal(bits(got), bits(O.sort_choose(x, k)))
@@ -108,24 +118,36 @@ def test_select_invariants(pkg, O, case):
     assert int(chunks["len"].sum()) == n
     rows = read(lib, g.rows + 4 * int(sd["row_base"]), np.uint32, nch * int(sd["bmax"]))
     rows = rows.reshape(nch, int(sd["bmax"]))
+    words = read(lib, g.bits + 4 * int(sd["bits_off"]), np.uint32, (B + 31) // 32)
+    active = np.nonzero(np.unpackbits(words.view(np.uint8), bitorder="little"))[0]
+    # the compaction keeps the first 4096 children's cursors in LDS (1024
+    # for 8-byte keys); the later children's keys are placed through their
+    # chunk's row, which so ends advanced by the chunk's keys in that bucket
+    late = active[(4096 if x.itemsize == 4 else 1024):]
     run = np.zeros(B, np.int64)
     for c in chunks[np.argsort(chunks["idx"])]:
-        assert np.array_equal(rows[int(c["idx"]), :B], run), f"row of chunk {int(c['idx'])}"
         s0 = int(c["start"])
-        run += np.bincount(b[s0:s0 + int(c["len"])], minlength=B)
+        h = np.bincount(b[s0:s0 + int(c["len"])], minlength=B)
+        want = run.copy()
+        want[late] += h[late]
+        assert np.array_equal(rows[int(c["idx"]), :B], want), f"row of chunk {int(c['idx'])}"
+        run += h
     assert np.array_equal(run, counts)
 
     # children: the active buckets in order, packed, exact
     U = int(st["nact"])
     kids = read(lib, g.children + CHILD.itemsize * int(sd["q_off"]), CHILD, U)
-    words = read(lib, g.bits + 4 * int(sd["bits_off"]), np.uint32, (B + 31) // 32)
-    active = np.nonzero(np.unpackbits(words.view(np.uint8), bitorder="little"))[0]
     assert np.array_equal(kids["bucket"].astype(np.int64), active)
     assert np.all(kids["src"] == 1)
     assert np.array_equal(kids["len"].astype(np.int64), counts[active])
     ca = counts[active]
     starts = int(sd["red_base"]) + np.cumsum(ca) - ca  # exclusive prefix
     assert np.array_equal(kids["start"].astype(np.int64), starts.astype(np.int64))
+    # the compaction's dense rounds: at least a quarter of the keys active
+    # (and at most 1024 children)
+    assert int(st["dense"]) == int(U <= 1024 and 4 * int(ca.sum()) >= n), (int(st["dense"]), U)
+    if "dense" in label:
+        assert int(st["dense"]) == 1
     red = read(lib, g.red + x.itemsize * int(sd["red_base"]), x.dtype, int(counts[active].sum()))
     below = np.concatenate([[0], np.cumsum(counts)])  # keys below each bucket
     order = np.argsort(b, kind="stable")  # keys grouped by bucket
