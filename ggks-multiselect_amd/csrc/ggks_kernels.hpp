@@ -1778,7 +1778,22 @@ __device__ __forceinline__ void compact_chunk(const KT* __restrict__ src,
           }
         }
         __syncthreads();  // the round is sorted
-        for (uint32_t j = tid; j < tot; j += kThreads) rseg[dl[su[j]] + j] = srt[j];
+        // four slots per thread per step, so their LDS round trips overlap
+        uint32_t j = tid;
+        for (; j + 3 * kThreads < tot; j += 4 * kThreads) {
+          uint32_t uu[4], dd[4];
+          KT kk[4];
+#pragma unroll
+          for (int f = 0; f < 4; ++f) {
+            uu[f] = su[j + f * kThreads];
+            kk[f] = srt[j + f * kThreads];
+          }
+#pragma unroll
+          for (int f = 0; f < 4; ++f) dd[f] = dl[uu[f]];
+#pragma unroll
+          for (int f = 0; f < 4; ++f) rseg[dd[f] + j + f * kThreads] = kk[f];
+        }
+        for (; j < tot; j += kThreads) rseg[dl[su[j]] + j] = srt[j];
       }
     }
   };
