@@ -1639,7 +1639,119 @@ __device__ __forceinline__ void compact_chunk(const KT* __restrict__ src,
   // every thread, with the same n: place the buffered keys, empty the buffer.
   // Buffered keys are active by construction; F of them per thread go
   // through each step together so their LDS round trips overlap.
+  // Sorted placement (full instance, more than kDenseChildren children, at
+  // most SORT_MAX keys): each key takes
+  // its slot from its child's cursor as below, and the keys are then
+  // written to LDS in child order and stored from there, so consecutive
+  // lanes store consecutive slots of a child and a wavefront's stores cover
+  // a few lines instead of one line per key.  The cursors at the flush's
+  // start (read by the threads owning them) and the children's counts in
+  // this flush give each child's first sorted slot; while the sorted keys
+  // are stored, a child's cursor word holds (its start - its first slot).
+  // The sorted keys and their children sit in the buffer behind the
+  // flushed keys.
+  constexpr uint32_t SORT_MAX =
+      (CAP * sizeof(KT) / (2 * sizeof(KT) + 2)) / kThreads * kThreads;
+  auto sorted_place = [&](uint32_t n) {
+    if constexpr (!SMALL) {
+      constexpr int FS = SORT_MAX / kThreads;
+      constexpr uint32_t CPT = CURS / kThreads;  // cursors per owning thread
+      constexpr uint32_t NONE32 = 0xFFFFFFFFu;
+      static_assert(FS >= 1 && CURS % kThreads == 0 && CURS <= 65536, "sorted flush shape");
+      KT* const srt = kb + SORT_MAX;
+      uint16_t* const su = reinterpret_cast<uint16_t*>(kb + 2 * SORT_MAX);
+      static_assert(2 * SORT_MAX * sizeof(KT) + 2 * SORT_MAX <= CAP * sizeof(KT), "sorted flush fits");
+      const uint32_t UC = U < CURS ? U : CURS, wid = tid >> 6;
+      uint32_t cb[CPT];
+#pragma unroll
+      for (uint32_t i = 0; i < CPT; ++i) {
+        const uint32_t u = CPT * tid + i;
+        cb[i] = u < UC ? cur[u] : 0u;
+      }
+      __syncthreads();  // the cursors are read before they advance
+      KT kk[FS];
+      uint32_t bb[FS], pp[FS];
+#pragma unroll
+      for (int f = 0; f < FS; ++f) kk[f] = kb[min(tid + f * kThreads, n - 1)];
+      bucket_batch<KT, C, FS>(bk, lut, kk, bb);
+#pragma unroll
+      for (int f = 0; f < FS; ++f) {
+        const uint32_t w = abits[bb[f] >> 5];
+        const uint32_t u = wbase[bb[f] >> 5] + __popc(w & ((1u << (bb[f] & 31u)) - 1u));
+        bb[f] = NONE32;
+        if (tid + f * kThreads < n) {
+          if (u < CURS) {
+            pp[f] = atomicAdd(&cur[u], 1u);
+            bb[f] = u;
+          } else {
+            rseg[kinfo[u].x + atomicAdd(&row[kinfo[u].y], 1u)] = kk[f];
+          }
+        }
+      }
+      __syncthreads();  // the cursors have advanced by this flush's keys
+      uint32_t c[CPT], sum = 0;
+#pragma unroll
+      for (uint32_t i = 0; i < CPT; ++i) {
+        const uint32_t u = CPT * tid + i;
+        c[i] = u < UC ? cur[u] - cb[i] : 0u;
+        sum += c[i];
+      }
+      const uint32_t inc = wave_incl_scan(sum);
+      if (lane_id() == 63) dsum[wid] = inc;
+      __syncthreads();  // wavefront totals written
+      const uint32_t ws = lane_id() < kThreads / 64 ? dsum[lane_id()] : 0u;
+      const uint32_t wi = wave_incl_scan(ws);
+      uint32_t t = (uint32_t)__builtin_amdgcn_readlane((int)(wi - ws), (int)wid) + inc - sum;
+      const uint32_t tot = (uint32_t)__builtin_amdgcn_readlane((int)wi, kThreads / 64 - 1);
+#pragma unroll
+      for (uint32_t i = 0; i < CPT; ++i) {
+        const uint32_t u = CPT * tid + i;
+        if (u < UC) cur[u] = cb[i] - t;
+        t += c[i];
+      }
+      __syncthreads();  // (start - first slot) of every child written
+#pragma unroll
+      for (int f = 0; f < FS; ++f) {
+        if (bb[f] != NONE32) {
+          const uint32_t j = pp[f] - cur[bb[f]];
+          srt[j] = kk[f];
+          su[j] = (uint16_t)bb[f];
+        }
+      }
+      __syncthreads();  // the flush is sorted
+      uint32_t j = tid;
+      for (; j + 3 * kThreads < tot; j += 4 * kThreads) {
+        uint32_t uu[4], dd[4];
+        KT ks[4];
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+          uu[f] = su[j + f * kThreads];
+          ks[f] = srt[j + f * kThreads];
+        }
+#pragma unroll
+        for (int f = 0; f < 4; ++f) dd[f] = cur[uu[f]];
+#pragma unroll
+        for (int f = 0; f < 4; ++f) rseg[dd[f] + j + f * kThreads] = ks[f];
+      }
+      for (; j < tot; j += kThreads) rseg[cur[su[j]] + j] = srt[j];
+      __syncthreads();  // every store has read its child's word
+#pragma unroll
+      for (uint32_t i = 0; i < CPT; ++i) {
+        const uint32_t u = CPT * tid + i;
+        if (u < UC) cur[u] = cb[i] + c[i];
+      }
+    }
+  };
   auto flush = [&](uint32_t n) {
+    // (sorting pays with many children; at 1024 k of a 2^28 N(0,1) vector
+    // it costs 6 us, at 4096 k it saves 70 us)
+    if (!SMALL && U > kDenseChildren && n <= SORT_MAX) {  // uniform
+      if (n) sorted_place(n);
+      __syncthreads();
+      if (tid == 0) nbuf = 0;
+      __syncthreads();
+      return;
+    }
     constexpr int F = SMALL ? 4 : 8;
     uint32_t i = tid;
     for (; i + (F - 1) * kThreads < n; i += F * kThreads) {
