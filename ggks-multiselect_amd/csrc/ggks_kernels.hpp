@@ -1445,7 +1445,7 @@ __device__ __forceinline__ void findk_segment(const SegDesc* __restrict__ segs,
   }
   if (tid == 0) {
     states[blockIdx.x].nact = U;
-    states[blockIdx.x].dense = (U <= kDenseChildren && ((uint64_t)mtot << kDenseLog2) >= sd.len) ? 1u : 0u;
+    states[blockIdx.x].nred = mtot;
     if (bigw && lbig) atomicAdd(bigw, lbig);
   }
 }
@@ -1521,7 +1521,7 @@ __host__ __device__ constexpr size_t compact_lds_bytes() {
   return CompactCfg<SMALL>::dyn_bytes;
 }
 
-template <typename KT, int C, bool SMALL>
+template <typename KT, int C, bool SMALL, bool WIN>
 __device__ __forceinline__ void compact_chunk(const KT* __restrict__ src,
                                               const SegDesc* __restrict__ segs,
                                               const SegState* __restrict__ states,
@@ -1571,7 +1571,7 @@ __device__ __forceinline__ void compact_chunk(const KT* __restrict__ src,
   if (head > len) head = len;
   const uint64_t nvec = (len - head) / VEC;
   const uint64_t done = head + nvec * VEC;
-  const uint4* __restrict__ pv = reinterpret_cast<const uint4*>(p + head);
+  const uint4* pv = reinterpret_cast<const uint4*>(p + head);
   const uint64_t per_round = (uint64_t)kRound * kThreads;
   const uint64_t rounds = (nvec + per_round - 1) / per_round;  // uniform
   const uint64_t full_rounds = nvec / per_round;  // rounds with every load in range
@@ -1596,42 +1596,69 @@ __device__ __forceinline__ void compact_chunk(const KT* __restrict__ src,
       v[t] = ok ? stream_load(q + (uint64_t)t * kThreads) : make_uint4(0, 0, 0, 0);
     }
   };
-  // round 0's loads go out before the tables are loaded and the cursors
-  // set, so they arrive during that prologue
-  if (rounds) load_round(0);
-  strided_for(lut_pool + sd.lut_off, st.P, [&](uint32_t i, uint32_t x) { lut[i] = x; });
-  for (uint32_t i = tid; i < nwords; i += kThreads) {
-    const uint32_t w = bits_pool[sd.bits_off + i];
-    abits[i] = w;
-    wbase[i] = wbase_pool[sd.bits_off + i];
-    // bits 4j..4j+3 -> bytes of abyte[8i + j]: x * 0x204081 puts bit m of a
-    // nibble x at bit 8m, and no two shifted copies overlap
-#pragma unroll
-    for (uint32_t j = 0; j < 8; ++j) abyte[i * 8 + j] = (((w >> (4 * j)) & 0xFu) * 0x204081u) & 0x01010101u;
-  }
-  const uint8_t* const act_of = reinterpret_cast<const uint8_t*>(abyte);
-  uint32_t* row = rows + sd.row_base + (uint64_t)ch.idx * sd.bmax;
   const uint2* kinfo = cinfo + sd.q_off;  // per child: slice start in the segment's part, bucket
+  uint32_t* row = rows + sd.row_base + (uint64_t)ch.idx * sd.bmax;
   // Cursors are positions inside the segment's part of the reduced buffer
   // (child start - red_base < segment length < 2^32), so they stay 32-bit
   // however far into the buffer the segment sits (batches and served groups
   // of more than 2^32 keys).
   KT* const rseg = red + sd.red_base;
-  for (uint32_t u = tid; u < U && u < CURS; u += kThreads)
-    cur[u] = kinfo[u].x + row[kinfo[u].y];
-  if (tid == 0) nbuf = 0;
+  // Cursor windows.  A segment of more children than LDS cursors either
+  // places the keys of the children beyond through global atomics on the
+  // chunk's row, or streams the chunk once per window of CURS children,
+  // each pass placing only its window's keys.  A key placed through a
+  // global atomic costs about what streaming 200 bytes does (at 2^28 f32
+  // keys, 16384 k: ~94 M such keys took ~2.9 ms), so the windows are taken
+  // when those keys times 200 bytes exceed the extra passes' bytes.
+  constexpr uint64_t kWindowKeyBytes = 200;
+  uint32_t nwin = 1;
+  if (WIN && U > CURS) {  // uniform
+    const uint64_t beyond = st.nred - kinfo[CURS].x;  // keys of children CURS..U-1
+    const uint32_t w = (U + CURS - 1) / CURS;
+    if (beyond * kWindowKeyBytes > (uint64_t)(w - 1) * sd.len * sizeof(KT)) nwin = w;
+  }
+  uint32_t wlo = 0;                       // the window's first child
+  uint32_t ucur = U < CURS ? U : CURS;    // its children with an LDS cursor
+  const uint8_t* const act_of = reinterpret_cast<const uint8_t*>(abyte);
+  // the window's tables: the byte table of its active buckets, its cursors
+  auto window_tables = [&]() {
+    for (uint32_t i = tid; i < nwords; i += kThreads) {
+      uint32_t w = abits[i];
+      if (nwin > 1) {  // uniform: only the bits of children in [wlo, wlo + ucur)
+        uint32_t c = wbase[i], m = 0;
+        for (uint32_t mm = w; mm; mm &= mm - 1, ++c)
+          if (c >= wlo && c < wlo + ucur) m |= mm & (0u - mm);
+        w = m;
+      }
+      // bits 4j..4j+3 -> bytes of abyte[8i + j]: x * 0x204081 puts bit m of a
+      // nibble x at bit 8m, and no two shifted copies overlap
+#pragma unroll
+      for (uint32_t j = 0; j < 8; ++j) abyte[i * 8 + j] = (((w >> (4 * j)) & 0xFu) * 0x204081u) & 0x01010101u;
+    }
+    for (uint32_t u = tid; u < ucur; u += kThreads)
+      cur[u] = kinfo[wlo + u].x + row[kinfo[wlo + u].y];
+    if (tid == 0) nbuf = 0;
+  };
+  // round 0's loads go out before the tables are loaded and the cursors
+  // set, so they arrive during that prologue
+  if (rounds) load_round(0);
+  strided_for(lut_pool + sd.lut_off, st.P, [&](uint32_t i, uint32_t x) { lut[i] = x; });
+  for (uint32_t i = tid; i < nwords; i += kThreads) {
+    abits[i] = bits_pool[sd.bits_off + i];
+    wbase[i] = wbase_pool[sd.bits_off + i];
+  }
+  window_tables();  // (each thread reads back only the words it wrote)
   __syncthreads();
   BucketFn<KT> bk;
   bk.init(st);
   // moves a key of an active bucket to its child's slice
   auto place = [&](KT raw) {
     const uint32_t b = bk(to_key<C>(raw), lut);
-    const uint32_t bits = abits[b >> 5];
-    const uint32_t sh = b & 31u;
-    if ((bits >> sh) & 1u) {
-      const uint32_t u = wbase[b >> 5] + __popc(bits & ((1u << sh) - 1u));
-      const uint32_t pos = u < CURS
-                               ? atomicAdd(&cur[u], 1u)
+    if (act_of[b]) {
+      const uint32_t bits = abits[b >> 5];
+      const uint32_t u = wbase[b >> 5] + __popc(bits & ((1u << (b & 31u)) - 1u));
+      const uint32_t pos = u - wlo < CURS
+                               ? atomicAdd(&cur[u - wlo], 1u)
                                : kinfo[u].x + atomicAdd(&row[kinfo[u].y], 1u);
       rseg[pos] = raw;
     }
@@ -1661,7 +1688,7 @@ __device__ __forceinline__ void compact_chunk(const KT* __restrict__ src,
       KT* const srt = kb + SORT_MAX;
       uint16_t* const su = reinterpret_cast<uint16_t*>(kb + 2 * SORT_MAX);
       static_assert(2 * SORT_MAX * sizeof(KT) + 2 * SORT_MAX <= CAP * sizeof(KT), "sorted flush fits");
-      const uint32_t UC = U < CURS ? U : CURS, wid = tid >> 6;
+      const uint32_t UC = ucur, wid = tid >> 6;
       uint32_t cb[CPT];
 #pragma unroll
       for (uint32_t i = 0; i < CPT; ++i) {
@@ -1680,9 +1707,9 @@ __device__ __forceinline__ void compact_chunk(const KT* __restrict__ src,
         const uint32_t u = wbase[bb[f] >> 5] + __popc(w & ((1u << (bb[f] & 31u)) - 1u));
         bb[f] = NONE32;
         if (tid + f * kThreads < n) {
-          if (u < CURS) {
-            pp[f] = atomicAdd(&cur[u], 1u);
-            bb[f] = u;
+          if (u - wlo < CURS) {
+            pp[f] = atomicAdd(&cur[u - wlo], 1u);
+            bb[f] = u - wlo;
           } else {
             rseg[kinfo[u].x + atomicAdd(&row[kinfo[u].y], 1u)] = kk[f];
           }
@@ -1766,8 +1793,8 @@ __device__ __forceinline__ void compact_chunk(const KT* __restrict__ src,
       for (int f = 0; f < F; ++f) u[f] = wbase[bb[f] >> 5] + __popc(w[f] & ((1u << (bb[f] & 31u)) - 1u));
 #pragma unroll
       for (int f = 0; f < F; ++f) {
-        const uint32_t pos = u[f] < CURS
-                                 ? atomicAdd(&cur[u[f]], 1u)
+        const uint32_t pos = u[f] - wlo < CURS
+                                 ? atomicAdd(&cur[u[f] - wlo], 1u)
                                  : kinfo[u[f]].x + atomicAdd(&row[kinfo[u[f]].y], 1u);
         rseg[pos] = kk[f];
       }
@@ -1778,8 +1805,10 @@ __device__ __forceinline__ void compact_chunk(const KT* __restrict__ src,
     __syncthreads();
   };
   // head and tail: fewer than VEC keys each (a thread may own one of each)
-  if (tid < head) place(p[tid]);
-  if (done + tid < len) place(p[done + tid]);
+  auto edges = [&]() {
+    if (tid < head) place(p[tid]);
+    if (done + tid < len) place(p[done + tid]);
+  };
   auto main_loop = [&](auto left, auto hvt) {
     for (uint64_t r = 0; r < rounds; ++r) {
       KT k[N];
@@ -1909,7 +1938,11 @@ __device__ __forceinline__ void compact_chunk(const KT* __restrict__ src,
       }
     }
   };
-  if (!SMALL && st.dense) {  // uniform
+  // dense: at least a quarter of the keys active, at most 1024 children
+  // (so one cursor window)
+  const bool dense = U <= kDenseChildren && ((uint64_t)st.nred << kDenseLog2) >= sd.len;
+  if (!SMALL && dense) {  // uniform
+    edges();
     if (!st.hvon) {
       if (st.s <= 24) dense_loop(std::true_type{}, std::false_type{});
       else dense_loop(std::false_type{}, std::false_type{});
@@ -1919,17 +1952,42 @@ __device__ __forceinline__ void compact_chunk(const KT* __restrict__ src,
     }
     return;
   }
-  // uniform: the sub-field shift direction and the heavy-value term
-  if (!st.hvon) {
-    if (st.s <= 24) main_loop(std::true_type{}, std::false_type{});
-    else main_loop(std::false_type{}, std::false_type{});
+  // one pass over the chunk: edges, rounds, the last flush
+  auto pass = [&]() {
+    edges();
+    // uniform: the sub-field shift direction and the heavy-value term
+    if (!st.hvon) {
+      if (st.s <= 24) main_loop(std::true_type{}, std::false_type{});
+      else main_loop(std::false_type{}, std::false_type{});
+    } else {
+      if (st.s <= 24) main_loop(std::true_type{}, std::true_type{});
+      else main_loop(std::false_type{}, std::true_type{});
+    }
+    const uint32_t nb = nbuf;  // stable: nothing appends any more
+    __syncthreads();
+    flush(nb);  // ends with a barrier: every key of the pass is placed
+  };
+  if constexpr (!WIN) {
+    pass();
   } else {
-    if (st.s <= 24) main_loop(std::true_type{}, std::true_type{});
-    else main_loop(std::false_type{}, std::true_type{});
+    for (uint32_t win = 0;;) {  // uniform
+      // the chunk's address taken afresh in each window (an empty asm the
+      // compiler cannot see through), so per-lane load addresses are not
+      // kept live from one window to the next
+      {
+        uint64_t a = (uint64_t)(uintptr_t)pv;
+        asm volatile("" : "+s"(a));
+        pv = reinterpret_cast<const uint4*>((uintptr_t)a);
+      }
+      pass();
+      if (++win == nwin) break;
+      wlo = win * CURS;
+      ucur = U - wlo < CURS ? U - wlo : CURS;
+      if (rounds) load_round(0);
+      window_tables();
+      __syncthreads();
+    }
   }
-  const uint32_t nb = nbuf;  // stable: nothing appends any more
-  __syncthreads();
-  flush(nb);
 }
 
 #define GGKS_COMPACT_PARAMS                                                                   \
@@ -1940,15 +1998,22 @@ __device__ __forceinline__ void compact_chunk(const KT* __restrict__ src,
       const uint2 *__restrict__ cinfo, KT *__restrict__ red
 template <typename KT, int C>
 __global__ __launch_bounds__(kThreads) void k_compact(GGKS_COMPACT_PARAMS) {
-  compact_chunk<KT, C, false>(src, segs, states, chunks, lut_pool, rows, bits_pool, wbase_pool,
-                              cinfo, red);
+  compact_chunk<KT, C, false, false>(src, segs, states, chunks, lut_pool, rows, bits_pool,
+                                     wbase_pool, cinfo, red);
+}
+// with cursor windows: for plans with a segment of more ranks than LDS
+// cursors (a separate instance, so the window loop costs the others nothing)
+template <typename KT, int C>
+__global__ __launch_bounds__(kThreads) void k_compact_win(GGKS_COMPACT_PARAMS) {
+  compact_chunk<KT, C, false, true>(src, segs, states, chunks, lut_pool, rows, bits_pool,
+                                    wbase_pool, cinfo, red);
 }
 // at most 64 VGPRs: two 1024-thread work-groups per CU
 template <typename KT, int C>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) void k_compact_small(
     GGKS_COMPACT_PARAMS) {
-  compact_chunk<KT, C, true>(src, segs, states, chunks, lut_pool, rows, bits_pool, wbase_pool,
-                             cinfo, red);
+  compact_chunk<KT, C, true, false>(src, segs, states, chunks, lut_pool, rows, bits_pool,
+                                    wbase_pool, cinfo, red);
 }
 #undef GGKS_COMPACT_PARAMS
 

@@ -73,6 +73,7 @@ struct Plan {
   std::vector<Child> direct_children;
   bool direct = false;  // single-vector plan answered by the finisher alone
   uint32_t nq = 0;
+  uint32_t qmax = 0;  // most ranks of one engine segment
   uint32_t bmax_max = 0;
   uint32_t pbits_max = 0;
   // table sizes (equal to the vectors' sizes when the tables are built)
@@ -247,6 +248,7 @@ inline void plan_single(Plan& p, int kind, int dtype, uint64_t n, uint32_t nq,
   const int eb = dtype_bytes(dtype);
   const SingleShape sh = single_shape(kind, dtype, n, o);
   p.nq = nq;
+  p.qmax = nq;
   p.seed = o.seed ? o.seed : 0x6A09E667F3BCC909ull;
   size_t off = 0;
   auto take = [&](size_t bytes) {
@@ -486,6 +488,7 @@ inline bool plan_batch(Plan& p, int dtype, const std::vector<BatchSeg>& in, cons
     s.bits_off = (uint32_t)bitc;
     s.q_off = (uint32_t)qoff;
     s.q_cnt = g.num_k;
+    p.qmax = std::max(p.qmax, s.q_cnt);
     s.mode = mode;
     s.radix_bits = rb;
     s.qsrc = topk ? QSRC_TOPK : QSRC_K;

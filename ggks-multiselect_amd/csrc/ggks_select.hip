@@ -260,6 +260,7 @@ static void level_attrs(const Plan&) {
     set_lds_limit(k_findk_small<KT, C>, ((size_t)kFindkSmallBuckets + (1u << kMaxPrefixBits) +
                                          3 * ((kFindkSmallBuckets + 31) / 32) + 32) * 4);
     set_lds_limit(k_compact<KT, C>, compact_lds_bytes<false>());
+    set_lds_limit(k_compact_win<KT, C>, compact_lds_bytes<false>());
     set_lds_limit(k_compact_small<KT, C>, compact_lds_bytes<true>());
     set_lds_limit(k_finish<KT, C>, kFinishLdsTotal);
     return true;
@@ -404,6 +405,10 @@ static int enqueue_level(const Exec& x, const Plan& p, const KT* d_src, KT* d_ou
     if (compact_small(p.pbits_max, p.bmax_max))
       launch_stage(x.prof, 6, k_compact_small<KT, C>, dim3(nchunks), dim3(kThreads),
                    (uint32_t)compact_lds_bytes<true>(), s, d_src, segs, states, chunks, lut, rows,
+                   bits, wbase, cinfo, red);
+    else if (p.qmax > CompactCfg<false>::template cursors<KT>)  // cursor windows possible
+      launch_stage(x.prof, 6, k_compact_win<KT, C>, dim3(nchunks), dim3(kThreads),
+                   (uint32_t)compact_lds_bytes<false>(), s, d_src, segs, states, chunks, lut, rows,
                    bits, wbase, cinfo, red);
     else
       launch_stage(x.prof, 6, k_compact<KT, C>, dim3(nchunks), dim3(kThreads),

@@ -95,12 +95,13 @@ struct SegState {         // written on the device by the LUT kernel
   uint32_t hb;            // F's bucket of hv (L(hv) + 1) when hvon
   uint32_t nfree;         // next free child slot (second-level children)
   uint32_t arrive;        // LUT parts done (k_lut; zeroed by k_sample)
-  uint32_t dense;         // 1: most keys are active, compaction sorts rounds by child
+  uint32_t nred;          // keys of the active buckets (k_findk)
 };
 
 // A segment of at most kDenseChildren children whose active buckets hold at
-// least len / 2^kDenseLog2 keys is compacted by sorting each round's keys by
-// child in LDS and storing runs (the sort's tables hold kDenseChildren).
+// least len / 2^kDenseLog2 keys (dense) is compacted by sorting each round's
+// keys by child in LDS and storing runs (the sort's tables hold
+// kDenseChildren).
 constexpr uint32_t kDenseLog2 = 2;
 constexpr uint32_t kDenseChildren = 1024;
 

@@ -19,7 +19,7 @@ int main() {
   S(SegState); F(SegState, kb); F(SegState, kmax); F(SegState, lmax); F(SegState, hv);
   F(SegState, s); F(SegState, P); F(SegState, B); F(SegState, nact); F(SegState, poff);
   F(SegState, pwid); F(SegState, rsh); F(SegState, twid); F(SegState, lsh); F(SegState, hvon);
-  F(SegState, hb); F(SegState, nfree); F(SegState, arrive); F(SegState, dense);
+  F(SegState, hb); F(SegState, nfree); F(SegState, arrive); F(SegState, nred);
   S(Chunk); F(Chunk, seg); F(Chunk, idx); F(Chunk, start); F(Chunk, len);
   S(Child); F(Child, start); F(Child, len); F(Child, lo); F(Child, hi); F(Child, q_off);
   F(Child, q_cnt); F(Child, src); F(Child, need_range); F(Child, bucket); F(Child, seg);

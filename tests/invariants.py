@@ -25,7 +25,7 @@ SEGDESC = np.dtype([("data_off", "<u8"), ("len", "<u8"), ("row_base", "<u8"), ("
 SEGSTATE = np.dtype([("kb", "<u8"), ("kmax", "<u8"), ("lmax", "<u8"), ("hv", "<u8"), ("s", "<u4"),
                      ("P", "<u4"), ("B", "<u4"), ("nact", "<u4"), ("poff", "<u4"), ("pwid", "<u4"),
                      ("rsh", "<u4"), ("twid", "<u4"), ("lsh", "<u4"), ("hvon", "<u4"), ("hb", "<u4"),
-                     ("nfree", "<u4"), ("arrive", "<u4"), ("dense", "<u4")], align=True)
+                     ("nfree", "<u4"), ("arrive", "<u4"), ("nred", "<u4")], align=True)
 CHUNK = np.dtype([("seg", "<u4"), ("idx", "<u4"), ("start", "<u8"), ("len", "<u8")], align=True)
 CHILD = np.dtype([("start", "<u8"), ("len", "<u8"), ("lo", "<u8"), ("hi", "<u8"), ("q_off", "<u4"),
                   ("q_cnt", "<u4"), ("src", "<u4"), ("need_range", "<u4"), ("bucket", "<u4"),

@@ -12,8 +12,8 @@ fixes, not against a second copy of the kernels' arithmetic:
 - chunk rows are exclusive prefix sums over chunks of per-chunk histograms
   (the compaction then advances the rows of children past its LDS
   cursors by the chunk's keys of that child);
-- the segment is flagged dense (compacted by child-sorted rounds) exactly
-  when it has at most 1024 children holding at least a quarter of its keys;
+- the segment's reduced-key count (from which the compaction decides on
+  child-sorted rounds and cursor windows) is the active buckets' keys;
 - the children are the buckets of the active bitmap in order, packed
   contiguously, each with its exact count, a key range holding its keys
   (where one is planned), and a reduced slice equal to the multiset of its
@@ -55,6 +55,11 @@ CASES = [  # (label, dtype, family, lo, hi, kind, n[, number of k])
     # children's cursors in LDS, the rest place through the chunk rows
     ("u32-full-radix-4096k", np.uint32, "uniform", 0, 0xFFFFFFFF, "radix", 1 << 22, 4096),
     ("u32-full-radix-16384k", np.uint32, "uniform", 0, 0xFFFFFFFF, "radix", 1 << 22, 16384),
+    # 8-byte keys keep 1024 cursors in LDS: the further children take
+    # cursor windows (a pass over the chunk each) in the full compaction, the
+    # row atomics in the small one
+    ("f64-uniform-16384k-small", np.float64, "uniform", 0, 0, "bucket", 1 << 21, 16384),
+    ("f64-normal-16384k", np.float64, "normal", 0, 0, "bucket", 1 << 24, 16384),
 ]
 
 
@@ -121,9 +126,21 @@ def test_select_invariants(pkg, O, case):
     words = read(lib, g.bits + 4 * int(sd["bits_off"]), np.uint32, (B + 31) // 32)
     active = np.nonzero(np.unpackbits(words.view(np.uint8), bitorder="little"))[0]
     # the compaction keeps the first 4096 children's cursors in LDS (1024
-    # for 8-byte keys); the later children's keys are placed through their
-    # chunk's row, which so ends advanced by the chunk's keys in that bucket
-    late = active[(4096 if x.itemsize == 4 else 1024):]
+    # for 8-byte keys); the later children's keys are either placed through
+    # their chunk's row, which so ends advanced by the chunk's keys in that
+    # bucket, or (when those keys times 200 bytes outweigh re-streaming the
+    # chunk) by further passes over the chunk with the next cursor windows
+    # (the small compaction instance, for at most 4096 buckets and 2^10 LUT
+    # prefixes, keeps 1024 cursors and no windows)
+    small = (1 << int(sd["pbits"])) <= 1024 and int(sd["bmax"]) <= 4096
+    curs = 4096 if x.itemsize == 4 and not small else 1024
+    cnt_act = np.bincount(b, minlength=B)[active]
+    beyond = int(cnt_act[curs:].sum())
+    nwin = -(-active.size // curs)
+    windows = not small and active.size > curs and beyond * 200 > (nwin - 1) * n * x.itemsize
+    late = active[:0] if windows else active[curs:]
+    print(f"children {active.size}, small {small}, beyond the LDS cursors {beyond} keys, "
+          f"windows {bool(windows)}")
     run = np.zeros(B, np.int64)
     for c in chunks[np.argsort(chunks["idx"])]:
         s0 = int(c["start"])
@@ -143,11 +160,12 @@ def test_select_invariants(pkg, O, case):
     ca = counts[active]
     starts = int(sd["red_base"]) + np.cumsum(ca) - ca  # exclusive prefix
     assert np.array_equal(kids["start"].astype(np.int64), starts.astype(np.int64))
-    # the compaction's dense rounds: at least a quarter of the keys active
-    # (and at most 1024 children)
-    assert int(st["dense"]) == int(U <= 1024 and 4 * int(ca.sum()) >= n), (int(st["dense"]), U)
+    # the reduced-key count the compaction takes its dense rounds (at least a
+    # quarter of the keys active, at most 1024 children) and its cursor
+    # windows from
+    assert int(st["nred"]) == int(ca.sum())
     if "dense" in label:
-        assert int(st["dense"]) == 1
+        assert U <= 1024 and 4 * int(ca.sum()) >= n
     red = read(lib, g.red + x.itemsize * int(sd["red_base"]), x.dtype, int(counts[active].sum()))
     below = np.concatenate([[0], np.cumsum(counts)])  # keys below each bucket
     order = np.argsort(b, kind="stable")  # keys grouped by bucket
