@@ -152,9 +152,16 @@ def cpu_baseline(pkg, x: np.ndarray, k: np.ndarray, gpu_out: np.ndarray):
     }
 
 
+# Untimed selects before each side leg's timed steps.  A device that has
+# been idle (the CPU baseline runs between legs) takes ~40 selects to reach
+# its steady rate: 2^27 f64 timed after 3 warm-up calls ran 457-461 us per
+# select, after 40 ~444 us.
+LEG_WARMUP = 40
+
+
 def timed_selects(pkg, L, d, n, dk, dout, steps, kind="bucket", K=0, ctx=None, stages=None):
     """Mean wall time (ms) of `steps` stream-ordered selects between two
-    synchronisations, after three warm-up calls.  With a dict `stages`, a
+    synchronisations, after LEG_WARMUP warm-up calls.  With a dict `stages`, a
     short untimed pass afterwards fills it with per-stage microseconds (HIP
     events between the kernels, which add a little time of their own)."""
     lib = L.lib()
@@ -166,7 +173,7 @@ def timed_selects(pkg, L, d, n, dk, dout, steps, kind="bucket", K=0, ctx=None, s
             L.check(lib.ggks_topk_async(ctx.handle, code, d.ptr, n, K, dout.ptr, None))
         else:
             pkg.multiselect_async(d, dk, dout, ctx, kind=kind)
-    for _ in range(3):
+    for _ in range(LEG_WARMUP):
         one()
     L.check(lib.ggks_sync())
     t0 = time.perf_counter()
@@ -370,7 +377,7 @@ def f64_leg(pkg, L, steps: int):
     dk = pkg.DeviceArray.from_host(k)
     dout = pkg.DeviceArray(k.size, np.float64)
     ctx = pkg.Context(0)
-    for _ in range(3):
+    for _ in range(LEG_WARMUP):
         pkg.multiselect_async(d, dk, dout, ctx)
     L.check(lib.ggks_sync())
     t0 = time.perf_counter()
@@ -474,12 +481,15 @@ def main():
         pkg.multiselect_async(d, dk, dout, ctx, kind=a.kind, **opt)
     L.check(lib.ggks_sync())
     first = dout.to_host()
-    # stage breakdown: a short untimed pass with an event between every two
-    # kernels (each event adds a few microseconds of device time, so these
-    # stage times overstate the kernels slightly); it names the dominant
-    # streaming pass
+    # stage breakdown: an untimed pass of 40 selects with an event between
+    # every two kernels (each event adds a few microseconds of device time,
+    # so these stage times overstate the kernels slightly); it names the
+    # dominant streaming pass.  It also brings the device to its steady
+    # state before the timed region: on a fresh process the first ~40
+    # selects run 2-3 % slower (20 steps timed after 5 warm-up selects and a
+    # 10-select pass: 457-462 us; after 60 selects: 448 us, same box).
     per = L.NSTAGES + 1
-    nprof = max(3, min(a.steps, 10))
+    nprof = 40
     L.check(lib.ggks_ctx_profile_marks(ctx.handle, (1 << per) - 1))
     L.check(lib.ggks_ctx_profile_arm(ctx.handle, nprof))
     for _ in range(nprof):
