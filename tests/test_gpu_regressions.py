@@ -15,7 +15,9 @@
 - heavy values (a key repeated over 2 % .. 99.9 % of the vector: its own
   bucket) at the middle, smallest and largest keys, single and batched;
 - the radix digit window below the sampled keys' common prefix, with keys
-  outside it in the guard buckets.
+  outside it in the guard buckets;
+- children of many ranks (top-K and runs of consecutive k in one bucket,
+  sorted by the finisher), on both key widths.
 """
 import os
 import subprocess
@@ -286,3 +288,24 @@ def test_radix_digit_window(pkg, O, dtype):
         got = pkg.multiselect(d, k, kind="radix")
         assert np.array_equal(bits(got), bits(O.sort_choose(x, k))), (lo, hi)
         d.free()
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_many_rank_children(pkg, O, dtype):
+    """A bucket holding hundreds of the requested ranks: top-K and runs of
+    consecutive k (one run ending at the largest key) against the oracle."""
+    n = 1 << 24
+    _say(f"case many-rank children {np.dtype(dtype).name} n={n}")
+    d = pkg.generate(dtype, "normal", 91, n)
+    x = d.to_host()
+    ctx = pkg.Context()
+    for K in (33, 700, 4096):
+        got = pkg.topk(d, K, ctx=ctx)
+        want = O.sort_choose(x, np.arange(1, K + 1))
+        assert np.array_equal(bits(got), bits(want)), f"top-{K}"
+    runs = [np.arange(n // 3, n // 3 + 600), np.arange(1, 300), np.arange(n - 2000, n + 1)]
+    for kk in runs:
+        got = pkg.multiselect(d, kk, ctx=ctx)
+        assert np.array_equal(bits(got), bits(O.sort_choose(x, kk))), f"run at {kk[0]}"
+    ctx.close()
+    d.free()
