@@ -59,6 +59,35 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
   return v;
 }
 
+// LDS atomic add of 1 for every active lane on h[key], returning each
+// lane's old value.  When all active lanes of the wavefront name the same
+// key (all of top-K's ranks in one child), one lane adds their number and
+// the lanes take consecutive values, instead of 64 atomics on one address.
+__device__ __forceinline__ uint32_t lds_inc(uint32_t* h, uint32_t key) {
+  const uint64_t act = __ballot(1);
+  const uint32_t k0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)key);
+  if (__ballot(key == k0) == act) {  // uniform
+    const uint64_t below = act & ((1ull << (threadIdx.x & 63u)) - 1ull);
+    uint32_t base = 0;
+    if (below == 0) base = atomicAdd(&h[k0], (uint32_t)__popcll(act));
+    base = (uint32_t)__builtin_amdgcn_readfirstlane((int)base);
+    return base + (uint32_t)__popcll(below);
+  }
+  return atomicAdd(&h[key], 1u);
+}
+// LDS atomic OR of v into w[idx]; one atomic for the wavefront when every
+// active lane names the same word and value.
+__device__ __forceinline__ void lds_or(uint32_t* w, uint32_t idx, uint32_t v) {
+  const uint64_t act = __ballot(1);
+  const uint32_t i0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)idx);
+  const uint32_t v0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+  if (__ballot(idx == i0 && v == v0) == act) {  // uniform
+    if ((act & ((1ull << (threadIdx.x & 63u)) - 1ull)) == 0) atomicOr(&w[i0], v0);
+    return;
+  }
+  atomicOr(&w[idx], v);
+}
+
 // Value of lane 63 (the wavefront total after an inclusive scan), uniform.
 __device__ __forceinline__ uint32_t wave_last(uint32_t v) {
   return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
@@ -1275,7 +1304,7 @@ __device__ __forceinline__ void findk_segment(const SegDesc* __restrict__ segs,
       out[q.out] = from_key<C>(klo);
       qb[sd.q_off + i] = NONE;
     } else {
-      atomicOr(&bits[lo >> 5], 1u << (lo & 31));
+      lds_or(bits, lo >> 5, 1u << (lo & 31));
       qb[sd.q_off + i] = lo;
       qr[sd.q_off + i] = q.rank - hist[lo];
     }
@@ -1346,7 +1375,7 @@ __device__ __forceinline__ void findk_segment(const SegDesc* __restrict__ segs,
     if (b == NONE) continue;
     uint32_t w = b >> 5, mw = bits[w];
     uint32_t u = wb[w] + __popc(mw & ((1u << (b & 31)) - 1u));
-    atomicAdd(&hist[u], 1u);
+    lds_inc(hist, u);
     qb[sd.q_off + i] = u;
   }
   __syncthreads();
@@ -1431,7 +1460,7 @@ __device__ __forceinline__ void findk_segment(const SegDesc* __restrict__ segs,
   for (uint32_t i = tid; i < sd.q_cnt; i += kThreads) {
     uint32_t u = qb[sd.q_off + i];
     if (u == NONE) continue;
-    uint32_t pos = atomicAdd(&hist[u], 1u);
+    uint32_t pos = lds_inc(hist, u);
     Query q;
     q.rank = qr[sd.q_off + i];
     q.out = seg_out(sd, queries, i);
