@@ -1407,11 +1407,23 @@ __device__ __forceinline__ void findk_segment(const SegDesc* __restrict__ segs,
     if (!l2 && next - first > 1 && next - first <= kWaveRanks && clen <= kWaveMaxLen) {
       const uint32_t extra = next - first - 1;
       const uint32_t slot = U + atomicAdd(&lsplit, extra);
-      Child one = *cp;
-      one.q_cnt = 1;
-      for (uint32_t j = 0; j < extra; ++j) {
-        one.q_off = sd.q_off + first + 1 + j;
-        children[sd.q_off + slot + j] = one;
+      // the record copied as 8-byte words held in registers (a Child
+      // object here lived in scratch memory), q_off and q_cnt in word 4
+      static_assert(sizeof(Child) == 56 && offsetof(Child, q_off) == 32 &&
+                        offsetof(Child, q_cnt) == 36, "Child layout");
+      const uint2* cw = reinterpret_cast<const uint2*>(cp);
+      uint2 rec[7];
+#pragma unroll
+      for (int w = 0; w < 7; ++w) rec[w] = cw[w];
+      rec[4].y = 1;
+#pragma unroll
+      for (uint32_t j = 0; j < kWaveRanks - 1; ++j) {
+        if (j < extra) {
+          rec[4].x = sd.q_off + first + 1 + j;
+          uint2* dw = reinterpret_cast<uint2*>(&children[sd.q_off + slot + j]);
+#pragma unroll
+          for (int w = 0; w < 7; ++w) dw[w] = rec[w];
+        }
       }
       cp->q_cnt = 1;
     }
